@@ -1,0 +1,52 @@
+// Per-cell arithmetic of the FARGO substep, shared by the per-routine
+// kernels (fg_steps.hip) and the fused ring-band kernels (fg_fused.hip).
+//
+// Each function evaluates one reference expression in the reference's own
+// left-to-right order (the build uses -ffp-contract=off), so both paths
+// round exactly like the FP64 oracle.  Indices: l = i*ns + j, row-major.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define FG_DEV __device__ __forceinline__
+
+// van Leer radial slope, Kernels.cu:808-818
+FG_DEV double vl_rad_slope(double qm, double q0, double qp, double idrm_i, double idrm_ip) {
+  double dqm = (q0 - qm) * idrm_i;
+  double dqp = (qp - q0) * idrm_ip;
+  if (dqp * dqm > 0.0) return 2.0 * dqp * dqm / (dqp + dqm);
+  return 0.0;
+}
+
+// upwind interface value at Rinf[i], Kernels.cu:828-833.  Callers handle
+// the i == 0 and i == nrad rows, which the reference zeroes.
+FG_DEV double vl_rad_star(double vr, double dt, double q_im1, double q_i, double dq_im1, double dq_i,
+                          double rmed_im1, double rmed_i, double rmed_ip1, int i) {
+  if (vr > 0.0 && i > 0) return q_im1 + (rmed_i - rmed_im1 - vr * dt) * 0.5 * dq_im1;
+  return q_i - (rmed_ip1 - rmed_i + vr * dt) * 0.5 * dq_i;
+}
+
+// van Leer azimuthal slope, Kernels.cu:1140-1146
+FG_DEV double vl_theta_slope(double qm, double q0, double qp, double invdxtheta) {
+  double dqm = (q0 - qm);
+  double dqp = (qp - q0);
+  if (dqp * dqm > 0.0) return dqp * dqm / (dqp + dqm) * invdxtheta;
+  return 0.0;
+}
+
+// upwind interface value at the j-1/2 face, Kernels.cu:1164-1168
+FG_DEV double vl_theta_star(double v, double dt, double q_jm1, double q_j, double dq_jm1, double dq_j,
+                            double dxtheta) {
+  double ksi = v * dt;
+  if (ksi > 0.0) return q_jm1 + (dxtheta - ksi) * dq_jm1;
+  return q_j - (dxtheta + ksi) * dq_j;
+}
+
+// von Neumann-Richtmyer artificial pressure, Kernels.cu:420-432
+FG_DEV double art_visc_q(double dens, double dv) {
+  if (dv < 0.0) return 1.41 * 1.41 * dens * dv * dv;
+  return 0.0;
+}
+
+// evanescent relaxation, Kernels.cu:1006-1012
+FG_DEV double damp_toward(double q, double lambda, double q0) { return (q + lambda * q0) / (1.0 + lambda); }

@@ -1,0 +1,95 @@
+// Internal state of an fg_ctx: device buffers, tables and launch helpers.
+// Shared by the HIP translation units of libfargo_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fargo_hip.h"
+
+#define FG_PI 3.14159265358979323844
+#define FG_CVNR 1.41
+#define FG_CFLSECURITY 0.5
+// DeviceReduce (Kernels.cu:364-394) never uses more than 1024 blocks.
+#define FG_RED_BLOCKS 1024
+
+void fg_set_error(const char *fmt, ...);
+
+#define FG_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      fg_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return FG_E_HIP;                                                               \
+    }                                                                                \
+  } while (0)
+
+#define FG_LAUNCH_CHECK()                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess) {                                                               \
+      fg_set_error("%s:%d kernel launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+      return FG_E_HIP;                                                                    \
+    }                                                                                     \
+  } while (0)
+
+// Device copies of the host tables (fg_tables).
+struct fg_dev_tables {
+  double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *invRinf, *invSurf, *invRmed, *invdiffRsup, *invdiffRmed;
+  double *SigmaMed, *SigmaInf, *EnergyMed, *CoolingTimeMed, *QplusMed;
+  double *cs_iso, *supp_torque, *dxtheta, *invdxtheta, *dxtheta_cfl;
+  int *ev_zone;
+  double *ev_coef, *ev_period, *ev_vtheta_arg;
+  double *visc_const, *visc_pow15, *visc_cav;
+  int *visc_cav_on;
+  double *cos_cell, *sin_cell, *cos_pot, *sin_pot;
+  double *sg_Kr, *sg_Kt;
+  double *sg_norm_r, *sg_norm_t, *sg_sqrt_ratio, *sg_ratio;
+  double dtheta, invdphi, dt1d_coef, vrad0_slope, visc_value;
+};
+
+struct fg_planets_arg {
+  int nb;
+  double x[FG_MAX_PLANETS], y[FG_MAX_PLANETS], m[FG_MAX_PLANETS], smooth[FG_MAX_PLANETS],
+      invd3[FG_MAX_PLANETS];
+  double ix, iy;  // indirect term of the star (IndirectTerm.x/.y)
+};
+
+struct fg_ctx {
+  fg_params p;
+  int nr, ns;
+  long S;  // (nr+1)*ns
+  int device;
+  hipStream_t stream;
+  fg_dev_tables t;
+
+  // evolved state and per-substep fields, (nr+1)*ns doubles each
+  double *f[FG_NFIELDS];
+  // remaining work arrays of the per-routine path
+  double *DRR, *DPP, *DRP, *Work, *QRStar, *DensStar, *dq, *TempShift, *Vradial, *Vazimutal;
+  // ring arrays (nr+1)
+  double *Vmoy, *newDT, *DT1D, *VMed, *nu, *cs_mean, *sgacc_mean;
+  int *Nshift, *NoSplit;
+  double *LostByDisk;  // ns
+  // reduction scratch
+  double *red_partial;  // 2 * FG_RED_BLOCKS
+  double *red_out;      // [0..1] force, [2] lost mass, [3] spare
+  int *cfl_out;         // [0] CFL count
+  double *h_out;        // pinned host mirror of red_out (4 doubles) + CFL
+  int *h_cfl;
+  // self-gravity complex buffers: 2*nr x ns interleaved complex
+  double *sg_Sr, *sg_St, *sg_Krc, *sg_Ktc, *sg_Gr, *sg_Gt, *sg_tmp;
+  int sg_kernel_ready;
+  // FP32 bulk mirrors (precision mode)
+  float *f32[8];
+  // host copies of ring values the host-side parts of a routine need
+  std::vector<double> h_Rmed, h_Rinf, h_SigmaMed, h_EnergyMed, h_vtheta_arg;
+  double lr01, lr12;  // log(Rmed0/Rmed1), log(Rmed1/Rmed2), Kernels.cu:93-94
+  // fused-path scratch
+  double *fu_mom[4];  // transported radial momenta per ring band
+  double *fu_ring;    // per-ring partial results
+};
+
+static inline int fg_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -1,0 +1,247 @@
+/*
+ * fargo_hip.h -- C-ABI of the MI355X (gfx950) FARGO / FARGO-ADSG hydro
+ * substep library (libfargo_hip.so).
+ *
+ * The reference (MatiasFloresS/CudaProtoplanetaryDisks) has no plugin or
+ * FFI layer: its substep path is a set of host functions operating on
+ * file-scope device globals (include/SourceEuler.cuh:1-22,
+ * TransportEuler.cuh:1-19, SideEuler.cuh:1-15, Viscosity.cuh:1-7,
+ * SgMain.cuh:1-9, Pframeforce.cuh:1-8, Force.cuh:1-7).  Each entry point
+ * below replaces one of those functions; the comment names it (file:line).
+ * Differences from the reference boundary:
+ *   - state lives in an opaque fg_ctx (one per host thread, one HIP stream
+ *     each) instead of globals; the library owns all device memory;
+ *   - every call returns an fg_status; nothing calls exit();
+ *   - calls are stream-ordered; only downloads and fg_sync block.
+ * Layout of every 2D field: FP64 row-major [(NRAD+1) x NSEC], index
+ * i*NSEC + j with i the ring (i = NRAD is the ghost ring) and j the sector,
+ * exactly the reference's layout (Main.cu:203).
+ */
+#ifndef FARGO_HIP_H
+#define FARGO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int fg_status;
+#define FG_OK 0
+#define FG_E_INVALID 1     /* bad argument or NULL context */
+#define FG_E_HIP 2         /* HIP runtime error (see fg_last_error) */
+#define FG_E_UNSUPPORTED 3 /* option not implemented on this path */
+#define FG_E_PARAM 4       /* deck rejected (the reference exit(1)s) */
+#define FG_E_IO 5          /* file could not be read or written */
+#define FG_E_CRASH 6       /* negative or non-finite density/energy */
+#define FG_E_NOMEM 7       /* device or host allocation failed */
+
+/* planets per context (fg_potential) */
+#define FG_MAX_PLANETS 16
+
+#define FG_PREC_FP64 0 /* all fields and arithmetic in FP64 */
+#define FG_PREC_FP32 1 /* FP32 bulk fields; FP64 planets, residual, sums */
+
+typedef struct fg_ctx fg_ctx;
+
+/* Physics switches and constants a context needs.  Filled by the host
+ * driver from a parsed deck (fg_sim_create does this). */
+typedef struct fg_params {
+  int nrad, nsec;
+  int precision;        /* FG_PREC_* */
+  int adiabatic;        /* Interpret.cu:163-172 */
+  int cooling;          /* Interpret.cu:167 */
+  int self_gravity;     /* Interpret.cu:150-154 */
+  int viscosity_alpha;  /* Interpret.cu:184-187 */
+  int fast_transport;   /* Interpret.cu:132 (FARGO = 1) */
+  int open_inner;       /* Interpret.cu:133 */
+  int non_reflecting;   /* Interpret.cu:134 */
+  int evanescent;       /* Interpret.cu:135 */
+  int indirect_term;    /* Interpret.cu:149 */
+  int exclude_hill;     /* Interpret.cu:174 */
+  double adiabatic_index;
+  double alphaviscosity;
+  double sigmaslope;
+  double flaringindex;
+  double sg_eps, sg_rstep, sg_tstep; /* Main.cu:269-271 */
+} fg_params;
+
+/* Host-computed 1D tables.  Every transcendental the kernels need is
+ * evaluated once on the host (glibc) and uploaded, so device and host
+ * arithmetic see identical inputs.  Ring arrays have NRAD+1 entries,
+ * sector arrays NSEC entries. */
+typedef struct fg_tables {
+  /* FillPolar1DArrays, SourceEuler.cu:66-162 */
+  const double *Radii, *Rinf, *Rsup, *Rmed, *Surf;
+  const double *invRinf, *invSurf, *invRmed, *invdiffRsup, *invdiffRmed;
+  /* Theo.cu profiles */
+  const double *SigmaMed, *SigmaInf, *EnergyMed, *CoolingTimeMed, *QplusMed;
+  /* per-ring kernel constants (same expressions as the kernels) */
+  const double *cs_iso;      /* isothermal c_s, Kernels.cu:179 */
+  const double *supp_torque; /* Kernels.cu:40 */
+  const double *dxtheta;     /* 2 pi / NSEC * Rmed, Kernels.cu:41 */
+  const double *invdxtheta;  /* 1 / dxtheta */
+  const double *dxtheta_cfl; /* Rmed * 2 pi / NSEC, Kernels.cu:1295 */
+  /* evanescent damping, Kernels.cu:978-1004 */
+  const int *ev_zone;          /* 1 inside a wave-killing zone */
+  const double *ev_coef;       /* damping^2 * 10 */
+  const double *ev_period;     /* Tin or Tout */
+  const double *ev_vtheta_arg; /* G/r (1 - ...) argument of the sqrt */
+  /* FViscosity, Viscosity.cu:33-50 */
+  const double *visc_const; /* nu for constant viscosity, before cavity */
+  const double *visc_pow15; /* r^1.5 for alpha viscosity */
+  const int *visc_cav_on;   /* cavity factor applies */
+  const double *visc_cav;   /* cavity factor */
+  /* sector tables */
+  const double *cos_cell, *sin_cell; /* cos/sin((2 pi j)/NSEC), Kernels.cu:161 */
+  const double *cos_pot, *sin_pot;   /* cos/sin(j/NSEC*2 pi), Kernels.cu:1236 */
+  /* self-gravity kernel tables, 2*NRAD x NSEC (SgMain.cu:49-81) */
+  const double *sg_Kr, *sg_Kt;
+  /* scalars */
+  double dtheta;      /* 2 pi / NSEC */
+  double invdphi;     /* 1 / (2 M_PI / NSEC), Kernels.cu:742-743 */
+  double dt1d_coef;   /* 2 pi CFLSECURITY / NSEC, Kernels.cu:1332 */
+  double vrad0_slope; /* -SIGMASLOPE + .5, Kernels.cu:1002 */
+  double visc_value;  /* VISCOSITY */
+} fg_tables;
+
+/* one planet's contribution to the potential, as prepared by
+ * FillForcesArrays (Pframeforce.cu:68-78) */
+typedef struct fg_planet_pot {
+  double x, y;
+  double mass;   /* mass * MassTaper */
+  double smooth; /* smoothing length squared */
+  double inv_d3; /* 1 / |r_p|^3 */
+} fg_planet_pot;
+
+/* field identifiers for fg_get_field / fg_set_field */
+enum fg_field {
+  FG_DENS = 0, FG_VRAD, FG_VTHETA, FG_ENERGY, FG_PRESSURE, FG_SOUNDSPEED, FG_TEMPERATURE,
+  FG_POTENTIAL, FG_VRADINT, FG_VTHETAINT, FG_VRADNEW, FG_VTHETANEW, FG_ENERGYINT,
+  FG_ENERGYNEW, FG_DENSINT, FG_TEMPERINT, FG_QPLUS, FG_DIVV, FG_TAURR, FG_TAUPP, FG_TAURP,
+  FG_RADMOMP, FG_RADMOMM, FG_THETAMOMP, FG_THETAMOMM, FG_VTHETARES, FG_SG_ACCR, FG_SG_ACCT,
+  FG_NFIELDS
+};
+
+/* ---- context ---------------------------------------------------------- */
+/* Allocates device memory (Cudamalloc Main.cu:660, Computecudamalloc
+ * SourceEuler.cu:378, InitTransportDevice TransportEuler.cu:291,
+ * InitViscosityDevice Viscosity.cu:81, InitComputeAccelDevice
+ * SideEuler.cu:173) and uploads the tables. */
+fg_status fg_create(const fg_params *p, const fg_tables *t, int device, fg_ctx **out);
+/* FreeCuda, Main.cu:367 */
+fg_status fg_destroy(fg_ctx *ctx);
+const char *fg_last_error(void);
+fg_status fg_sync(fg_ctx *ctx);
+
+/* Host <-> device state.  NULL pointers are skipped.  Arrays are
+ * (NRAD+1)*NSEC doubles (Main.cu:811-825 / DeviceToHostcudaMemcpy
+ * Main.cu:605-616). */
+fg_status fg_upload_state(fg_ctx *ctx, const double *dens, const double *vrad, const double *vtheta,
+                          const double *energy);
+fg_status fg_download_state(fg_ctx *ctx, double *dens, double *vrad, double *vtheta, double *energy,
+                            double *temperature, double *qplus);
+fg_status fg_get_field(fg_ctx *ctx, int field, double *host);
+fg_status fg_set_field(fg_ctx *ctx, int field, const double *host);
+/* ring arrays (NRAD entries): Vmoy, VMed, viscosity, ring mean of c_s,
+ * ring mean of SG_Accr */
+fg_status fg_get_rings(fg_ctx *ctx, double *vmoy, double *vmed, double *viscosity, double *cs_mean,
+                       double *sgacc_mean);
+/* Nshift of the last transport (NRAD ints), ComputeConstantResidualKernel
+ * Kernels.cu:1114 */
+fg_status fg_get_nshift(fg_ctx *ctx, int *nshift);
+
+/* ---- the substep path, one entry per reference routine --------------- */
+/* ComputeSoundSpeed, SourceEuler.cu:475 */
+fg_status fg_sound_speed(fg_ctx *ctx);
+/* ComputePressureField, SourceEuler.cu:467 */
+fg_status fg_pressure(fg_ctx *ctx);
+/* ComputeTemperatureField, SourceEuler.cu:484 */
+fg_status fg_temperature(fg_ctx *ctx);
+/* Make1Dprofile(1) + FViscosity, Kernels.cu:626 / Viscosity.cu:33: ring
+ * mean of c_s and the viscosity profile derived from it */
+fg_status fg_ring_mean_cs(fg_ctx *ctx);
+/* ConditionCFL, SourceEuler.cu:520: writes the sub-cycle count */
+fg_status fg_cfl(fg_ctx *ctx, double DeltaT, int *ncfl);
+/* ComputeAccel, SideEuler.cu:135 (Force.cu:102): disk gravity on a point
+ * mass at (x, y) with smoothing rsmoothing, summed in DeviceReduce order
+ * (Kernels.cu:364).  out2 = {ax, ay}. */
+fg_status fg_disk_force(fg_ctx *ctx, double x, double y, double rsmoothing, double *out2);
+/* FillForcesArrays, Pframeforce.cu:56 (+ ComputeIndirectTerm :86) */
+fg_status fg_potential(fg_ctx *ctx, const fg_planet_pot *planets, int nb, double indirect_x,
+                       double indirect_y);
+/* CorrectVtheta, SideEuler.cu:181 */
+fg_status fg_correct_vtheta(fg_ctx *ctx, double domega);
+/* ApplyBoundaryCondition, SideEuler.cu:31 */
+fg_status fg_boundary(fg_ctx *ctx, double dt, double omega_frame);
+/* Substep1, SourceEuler.cu:311 (incl. self-gravity kick, viscous stress
+ * Viscosity.cu:52 and its velocity update Viscosity.cu:25, and the
+ * sub-Keplerian boundary SideEuler.cu:189) */
+fg_status fg_substep1(fg_ctx *ctx, double dt, double omega_frame);
+/* Substep2 + host + ActualiseGasVrad/Vtheta, SourceEuler.cu:345-358,
+ * 492-503 */
+fg_status fg_substep2(fg_ctx *ctx, double dt);
+/* ComputeViscousTerms + Substep3 + ActualiseGasEnergy,
+ * SourceEuler.cu:287-291, 362 */
+fg_status fg_substep3(fg_ctx *ctx, double dt);
+/* Transport, TransportEuler.cu:30; lost_mass (may be NULL) receives the
+ * mass that left through the open inner edge (TransportEuler.cu:77) */
+fg_status fg_transport(fg_ctx *ctx, double dt, double omega_frame, double *lost_mass);
+/* compute_selfgravity, SgMain.cu:23 (update = 0 evaluates only) */
+fg_status fg_selfgravity(fg_ctx *ctx, double dt, int update);
+
+/* ---- fused substep (same arithmetic, fewer passes over HBM) ---------- */
+/* Everything AlgoGas does on the GPU for one substep after the planets
+ * are advanced (SourceEuler.cu:255-295): CorrectVtheta, the three
+ * boundary passes, pressure, Substep1/2/3, transport and temperature.
+ * Also leaves the CFL sub-cycle count for DeltaT_next (the next substep's
+ * ConditionCFL, SourceEuler.cu:228) and the disk force on the star for it
+ * (SourceEuler.cu:241), fetched by fg_fused_results. */
+fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domega,
+                        const fg_planet_pot *planets, int nb, double indirect_x, double indirect_y);
+/* CFL count and star force of the current state (fused reductions) */
+fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
+fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);
+
+/* ---- host driver: the reference's main()/AlgoGas over this library ---- */
+typedef struct fg_sim fg_sim;
+typedef struct fg_sim_options {
+  int device;      /* HIP device ordinal */
+  int precision;   /* FG_PREC_* */
+  int fused;       /* 1 = fg_step_fused path, 0 = one call per routine */
+  int write_files; /* used_rad.dat, dims.dat, planet files, outputs */
+  const char *outdir;    /* overrides OUTPUTDIR when not NULL ("-o") */
+  const char *cli_flags; /* single-letter switches of Main.cu:113-179 */
+  unsigned long long noise_seed; /* 0 = none; else Sigma *= 1 + amp*xi */
+  double noise_amp;
+} fg_sim_options;
+
+/* Main.cu:187-305 */
+fg_status fg_sim_create(const char *parfile, const fg_sim_options *opt, fg_sim **out);
+fg_status fg_sim_destroy(fg_sim *sim);
+/* one iteration of AlgoGas's sub-cycle loop (SourceEuler.cu:222-305);
+ * *closed = 1 when it completed the DT */
+fg_status fg_sim_substep(fg_sim *sim, int *closed);
+/* AlgoGas, SourceEuler.cu:199 */
+fg_status fg_sim_algogas(fg_sim *sim);
+/* main()'s loop for it = it0..it1 (Main.cu:309-333), with outputs */
+fg_status fg_sim_run(fg_sim *sim, int it0, int it1);
+fg_status fg_sim_write_outputs(fg_sim *sim, int index);
+fg_status fg_sim_get_state(fg_sim *sim, double *dens, double *vrad, double *vtheta, double *energy);
+fg_status fg_sim_set_state(fg_sim *sim, const double *dens, const double *vrad, const double *vtheta,
+                           const double *energy);
+/* scalars: "PhysicalTime", "OmegaFrame", "LostMass", "dt", "dtemp",
+ * "MassTaper", "DT", "NTOT", "NINTERM", "NRAD", "NSEC", "last_cfl" */
+fg_status fg_sim_get(fg_sim *sim, const char *name, double *value);
+fg_status fg_sim_nplanets(fg_sim *sim, int *nb);
+/* out = {x, y, vx, vy, mass} */
+fg_status fg_sim_planet(fg_sim *sim, int k, double *out5);
+fg_ctx *fg_sim_ctx(fg_sim *sim);
+/* 1D grid of the run (NRAD+1 entries): "Radii", "Rinf", "Rsup", "Rmed",
+ * "Surf", "SigmaMed", "EnergyMed" */
+fg_status fg_sim_ring(fg_sim *sim, const char *name, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
