@@ -1,19 +1,867 @@
-// Fused substep path (filled in below the per-routine path).
+// Fused substep path: the arithmetic of the per-routine kernels in
+// fg_steps.hip (same expressions, same rounding), regrouped so that one
+// substep makes five passes over HBM instead of ~100:
+//
+//   k_src   2D tile (TH x TW cells + halo) staged in LDS:
+//           CorrectVtheta, evanescent damping, pressure, potential,
+//           Substep1, viscous stress, viscous update, artificial
+//           viscosity, velocity update, second damping pass
+//           state A -> state B
+//   k_trad  column-marching radial van Leer sweep of the five transported
+//           quantities (each slope and flux evaluated once)
+//           state B -> R (Sigma, RadMomP/M, ThetaMomP/M)
+//   k_taz   one ring per workgroup: VMed (sequential sum), FARGO residual
+//           and integer shift, both azimuthal sweeps, the shift, vtheta
+//           recovery; R -> X (Sigma, vtheta, RadMomP, RadMomM)
+//   k_fin   one ring per workgroup: vrad recovery, third damping pass,
+//           temperature, and the next substep's CFL ring minimum
+//           X -> state A
+//   force   DeviceReduce-order disk force on the star (fg_reduce.hip) and
+//           the CFL count (k_cfl_final in fg_steps.hip)
+//
+// Supported decks: locally isothermal, evanescent boundaries, no
+// self-gravity, per-DT CFL (everything else uses the per-routine path).
+#include "fg_cell.h"
 #include "fg_internal.h"
 
-extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
-                                   const fg_planet_pot *planets, int nb, double ix, double iy) {
-  (void)c; (void)dt; (void)omega_frame; (void)domega; (void)planets; (void)nb; (void)ix; (void)iy;
-  fg_set_error("fused path not built");
-  return FG_E_UNSUPPORTED;
+#define TH 16
+#define TW 64
+#define HALO_R 4  // rows above the tile (i - 4 .. )
+#define HALO_B 2  // rows below the tile ( .. i + TH + 1)
+#define HALO_C 4  // columns on each side
+#define HR (TH + HALO_R + HALO_B)
+#define HC (TW + 2 * HALO_C)
+#define SRC_TPB 256
+#define SRC_PER_THREAD ((HR * HC + SRC_TPB - 1) / SRC_TPB)
+
+fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
+
+__global__ void k_cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt, const double *invRmed,
+                            double coef, double DeltaT, int nr);
+
+struct fused_bufs {
+  double *Bd, *Bvr, *Bvt;                // after k_src
+  double *Rd, *Rrmp, *Rrmm, *Rtmp, *Rtmm;  // after k_trad
+  double *Xd, *Xvt, *Xrmp, *Xrmm;        // after k_taz
+  double *P;                             // pressure (temperature output only)
+};
+
+struct src_args {
+  const double *dens, *vr, *vt;
+  double *odens, *ovr, *ovt, *opress;
+  const double *nu;
+  fg_dev_tables t;
+  fg_planets_arg P;
+  double dt, omega_frame, domega;
+  int nr, ns, indirect, evanescent, corotating;
+};
+
+// potential at one cell (FillForcesArraysKernel, Kernels.cu:1234-1252)
+__device__ __forceinline__ double cell_potential(const fg_planets_arg &P, double Rm, double cosp, double sinp,
+                                                 int indirect) {
+  const double InvDistance = 1.0 / Rm;
+  const double x = Rm * cosp, y = Rm * sinp;
+  double acc = 0.0;
+  for (int k = 0; k < P.nb; k++) {
+    double xp = P.x[k], yp = P.y[k], m = P.m[k];
+    double distance = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+    double distancesmooth = sqrt(distance + P.smooth[k]);
+    double pv = -1.0 * m / distancesmooth;
+    if (indirect) pv += 1.0 * m * P.invd3[k] * (x * xp + y * yp);
+    acc += pv;
+    if (k == 0) {
+      pv = -1.0 * 1.0 * InvDistance;
+      pv -= P.ix * x + P.iy * y;
+      acc += pv;
+    }
+  }
+  return acc;
 }
+
+// evanescent damping constants of ring i for a step (Kernels.cu:978-1004)
+__device__ __forceinline__ void ev_consts(const fg_dev_tables &t, const double *nu, int i, double step,
+                                          double omega_frame, double &lambda, double &vrad0, double &vtheta0) {
+  const double Rm = t.Rmed[i];
+  lambda = t.ev_coef[i] * step / t.ev_period[i];
+  vtheta0 = sqrt(t.ev_vtheta_arg[i]);
+  vtheta0 -= Rm * omega_frame;
+  vrad0 = -3.0 * nu[i] / Rm * t.vrad0_slope;
+}
+
+__global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
+  __shared__ double sS[HR][HC];   // Sigma after the first damping
+  __shared__ double sA[HR][HC];   // pressure, then VradInt
+  __shared__ double sB[HR][HC];   // potential, then VthetaInt
+  __shared__ double sC[HR][HC];   // vtheta, then TAURR, then Qr
+  __shared__ double sD[HR][HC];   // vrad, then TAUPP, then Qtheta
+  __shared__ double sE[HR][HC];   // TAURP
+  const int nr = a.nr, ns = a.ns;
+  const int i0 = blockIdx.y * TH, j0 = blockIdx.x * TW;
+  const int tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const double dt = a.dt;
+#define GI(r) (i0 - HALO_R + (r))
+#define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
+
+  // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
+  for (int q = tid; q < HR * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    double s = 0.0, vr = 0.0, vt = 0.0, p = 0.0, phi = 0.0;
+    if (i >= 0 && i <= nr) {
+      int j = GJ(cc);
+      long l = (long)i * ns + j;
+      s = a.dens[l];
+      vr = a.vr[l];
+      vt = a.vt[l];
+      if (i < nr) {
+        if (a.corotating) vt = vt - a.domega * t.Rmed[i];
+        if (a.evanescent && t.ev_zone[i]) {
+          double lambda, vrad0, vtheta0;
+          ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
+          vr = damp_toward(vr, lambda, vrad0);
+          vt = damp_toward(vt, lambda, vtheta0);
+          s = damp_toward(s, lambda, t.SigmaMed[i]);
+        }
+        const double cs = t.cs_iso[i];
+        p = s * cs * cs;
+        phi = cell_potential(a.P, t.Rmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
+      }
+    }
+    sS[r][cc] = s;
+    sA[r][cc] = p;
+    sB[r][cc] = phi;
+    sC[r][cc] = vt;
+    sD[r][cc] = vr;
+  }
+  __syncthreads();
+
+  // ---- S1: Substep1 (Kernels.cu:13-53) on rows [1, HR), cols [1, HC-1) ----
+  double regA[SRC_PER_THREAD], regB[SRC_PER_THREAD];
+  {
+    int k = 0;
+    for (int q = tid; q < HR * HC; q += SRC_TPB, k++) {
+      int r = q / HC, cc = q - r * HC;
+      int i = GI(r);
+      double vri = 0.0, vti = 0.0;
+      if (r >= 1 && cc >= 1 && cc < HC - 1 && i >= 0 && i < nr) {
+        if (i > 0) {
+          double gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0 / (sS[r][cc] + sS[r - 1][cc]) * t.invdiffRmed[i];
+          double gradphi = (sB[r][cc] - sB[r - 1][cc]) * t.invdiffRmed[i];
+          double vt2 = sC[r][cc] + sC[r][cc + 1] + sC[r - 1][cc] + sC[r - 1][cc + 1];
+          vt2 = vt2 / 4.0 + a.omega_frame * t.Rinf[i];
+          vt2 = vt2 * vt2;
+          double vr_src = -gradp - gradphi;
+          double vradint2 = vr_src + vt2 * t.invRinf[i];
+          vri = sD[r][cc] + dt * vradint2;
+        }
+        const double invdxtheta = t.invdxtheta[i];
+        double gradp = (sA[r][cc] - sA[r][cc - 1]) * 2.0 / (sS[r][cc] + sS[r][cc - 1]) * invdxtheta;
+        double gradphi = (sB[r][cc] - sB[r][cc - 1]) * invdxtheta;
+        double v = sC[r][cc] - dt * (gradp + gradphi);
+        v += dt * t.supp_torque[i];
+        vti = v;
+      }
+      regA[k] = vri;
+      regB[k] = vti;
+    }
+    if (a.opress) {  // pressure for the temperature output, own cells only
+      for (int q = tid; q < TH * TW; q += SRC_TPB) {
+        int r = q / TW + HALO_R, cc = q % TW + HALO_C;
+        int i = GI(r), j = j0 + cc - HALO_C;
+        if (i < nr && j < ns) a.opress[(long)i * ns + j] = sA[r][cc];
+      }
+    }
+    __syncthreads();
+    k = 0;
+    for (int q = tid; q < HR * HC; q += SRC_TPB, k++) {
+      int r = q / HC, cc = q - r * HC;
+      sA[r][cc] = regA[k];  // VradInt (0 on rows 0 and NRAD, as never written)
+      sB[r][cc] = regB[k];  // VthetaInt
+    }
+  }
+  __syncthreads();
+
+  // ---- S2: viscous stress (Kernels.cu:733-771) on rows [1, HR-1), cols [2, HC-2) ----
+  const double invdphi = t.invdphi;
+  const double onethird = 1.0 / 3.0;
+  for (int q = tid; q < HR * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 1 || r >= HR - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
+    const double *VrR = &sA[r][0];
+    double drr = (sA[r + 1][cc] - VrR[cc]) * t.invdiffRsup[i];
+    double dpp = (sB[r][cc + 1] - sB[r][cc]) * invdphi * t.invRmed[i] + 0.5 * (sA[r + 1][cc] + VrR[cc]) * t.invRmed[i];
+    double dv = (sA[r + 1][cc] * t.Rsup[i] - VrR[cc] * t.Rinf[i]) * t.invdiffRsup[i] * t.invRmed[i];
+    dv += (sB[r][cc + 1] - sB[r][cc]) * invdphi * t.invRmed[i];
+    const double nu = a.nu[i];
+    sC[r][cc] = 2.0 * sS[r][cc] * nu * (drr - onethird * dv);
+    sD[r][cc] = 2.0 * sS[r][cc] * nu * (dpp - onethird * dv);
+    if (i > 0) {
+      double drp = 0.5 * (t.Rinf[i] * (sB[r][cc] * t.invRmed[i] - sB[r - 1][cc] * t.invRmed[i - 1]) * t.invdiffRmed[i] +
+                          (VrR[cc] - VrR[cc - 1]) * invdphi * t.invRinf[i]);
+      sE[r][cc] = 2.0 * 0.25 * (sS[r][cc] + sS[r - 1][cc] + sS[r][cc - 1] + sS[r - 1][cc - 1]) * nu * drp;
+    }
+  }
+  __syncthreads();
+
+  // ---- S3: viscous velocity update (Kernels.cu:118-150), in place ----
+  for (int q = tid; q < HR * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 2 || r >= HR - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
+    if (i < nr - 1 && r < HR - 2)
+      sB[r][cc] += dt * t.invRmed[i] *
+                   ((t.Rsup[i] * sE[r + 1][cc] - t.Rinf[i] * sE[r][cc]) * t.invdiffRsup[i] +
+                    (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5 * (sE[r][cc] + sE[r + 1][cc])) /
+                   (0.5 * (sS[r][cc] + sS[r][cc - 1]));
+    sA[r][cc] += dt * t.invRinf[i] *
+                 ((t.Rmed[i] * sC[r][cc] - t.Rmed[i - 1] * sC[r - 1][cc]) * t.invdiffRmed[i] +
+                  (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5 * (sD[r][cc] + sD[r - 1][cc])) /
+                 (0.5 * (sS[r][cc] + sS[r - 1][cc]));
+  }
+  __syncthreads();
+
+  // ---- S4: artificial pressure (Kernels.cu:409-434) on rows [3, HR-1), cols [3, HC-4) ----
+  for (int q = tid; q < HR * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 3 || r >= HR - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
+    sC[r][cc] = art_visc_q(sS[r][cc], sA[r + 1][cc] - sA[r][cc]);
+    sD[r][cc] = art_visc_q(sS[r][cc], sB[r][cc + 1] - sB[r][cc]);
+  }
+  __syncthreads();
+
+  // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2, store ----
+  for (int q = tid; q < TH * TW; q += SRC_TPB) {
+    int r = q / TW + HALO_R, cc = q % TW + HALO_C;
+    int i = GI(r), j = j0 + cc - HALO_C;
+    if (i > nr || j >= ns) continue;
+    long l = (long)i * ns + j;
+    if (i == nr) {  // ghost ring: VradNew/VthetaNew are never written (0)
+      a.odens[l] = sS[r][cc];
+      a.ovr[l] = 0.0;
+      a.ovt[l] = 0.0;
+      continue;
+    }
+    double vrnew = 0.0;
+    if (i > 0) {
+      double d = sS[r][cc] + sS[r - 1][cc];
+      double densint = sC[r][cc] - sC[r - 1][cc];
+      vrnew = sA[r][cc] - dt * 2.0 / d * densint * t.invdiffRmed[i];
+    }
+    const double invdxtheta = t.invdxtheta[i];
+    double dens2 = sS[r][cc] + sS[r][cc - 1];
+    double tempint = (sD[r][cc] - sD[r][cc - 1]);
+    double vtnew = sB[r][cc] - dt * 2.0 / dens2 * tempint * invdxtheta;
+    double s = sS[r][cc];
+    if (a.evanescent && t.ev_zone[i]) {
+      double lambda, vrad0, vtheta0;
+      ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
+      vrnew = damp_toward(vrnew, lambda, vrad0);
+      vtnew = damp_toward(vtnew, lambda, vtheta0);
+      s = damp_toward(s, lambda, t.SigmaMed[i]);
+    }
+    a.odens[l] = s;
+    a.ovr[l] = vrnew;
+    a.ovt[l] = vtnew;
+  }
+#undef GI
+#undef GJ
+}
+
+// ----------------------------------------------------------- k_trad ----
+// Radial sweep (OneWindRad, TransportEuler.cu:60-79) marching over rings
+// of a band; one thread owns one sector column.  For each ring m the
+// momenta (LRMomentaKernel), Work = Q / (Sigma + 1e-20), the slope and
+// the face value at face m-1 are formed once and kept in registers; the
+// face flux F(k) = dt*dtheta*Rinf[k]*Q*(k)*Sigma*(k)*vrad(k) serves both
+// neighbouring rings (Rsup[k-1] == Rinf[k]).
+#define TRAD_TPB 64
+#define TRAD_BAND 16
+#define NQ 5  // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Sigma
+
+struct trad_args {
+  const double *dens, *vr, *vt;
+  double *out[NQ];
+  fg_dev_tables t;
+  double dt, omega_frame;
+  int nr, ns;
+};
+
+__global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
+  const int nr = a.nr, ns = a.ns;
+  const int j = blockIdx.x * TRAD_TPB + threadIdx.x;
+  if (j >= ns) return;
+  const int jp = (j + 1) % ns;
+  const int b0 = blockIdx.y * TRAD_BAND;
+  const int b1 = min(b0 + TRAD_BAND, nr);
+  const fg_dev_tables &t = a.t;
+  const double dt = a.dt, OF = a.omega_frame;
+  const double cflux = dt * t.dtheta;
+  // rolling window: index 0 = ring m-2, 1 = m-1, 2 = m
+  double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
+  double sg[3] = {}, dqs[2] = {}, vr[3] = {};
+  int m0 = max(b0 - 2, 0);
+  for (int m = m0; m <= b1 + 1; m++) {
+    // shift the window
+    for (int q = 0; q < NQ; q++) {
+      Q[q][0] = Q[q][1];
+      Q[q][1] = Q[q][2];
+      W[q][0] = W[q][1];
+      W[q][1] = W[q][2];
+    }
+    sg[0] = sg[1];
+    sg[1] = sg[2];
+    vr[0] = vr[1];
+    vr[1] = vr[2];
+    if (m < nr) {
+      const long l = (long)m * ns + j;
+      const double s = a.dens[l];
+      const double v0 = a.vr[l], v1 = a.vr[l + ns];
+      const double Rm = t.Rmed[m];
+      Q[0][2] = s * v1;
+      Q[1][2] = s * v0;
+      Q[2][2] = s * (a.vt[(long)m * ns + jp] + Rm * OF) * Rm;
+      Q[3][2] = s * (a.vt[l] + Rm * OF) * Rm;
+      Q[4][2] = s;
+      sg[2] = s;
+      vr[2] = v0;
+      for (int q = 0; q < NQ; q++) W[q][2] = Q[q][2] / (s + 1e-20);
+    } else {
+      for (int q = 0; q < NQ; q++) Q[q][2] = W[q][2] = 0.0;
+      sg[2] = 0.0;
+      vr[2] = 0.0;  // vrad of the ghost ring
+    }
+    // slope of ring m-1 (zero on rings 0 and NRAD-1)
+    const int k = m - 1;
+    if (k >= 0) {
+      for (int q = 0; q < NQ; q++) {
+        dq[q][0] = dq[q][1];
+        dq[q][1] = (k == 0 || k >= nr - 1) ? 0.0 : vl_rad_slope(W[q][0], W[q][1], W[q][2], t.invdiffRmed[k], t.invdiffRmed[k + 1]);
+      }
+      dqs[0] = dqs[1];
+      dqs[1] = (k == 0 || k >= nr - 1) ? 0.0 : vl_rad_slope(sg[0], sg[1], sg[2], t.invdiffRmed[k], t.invdiffRmed[k + 1]);
+      // face k: upwind values (StarRadKernel2); faces 0 and NRAD are zero
+      double DS = 0.0;
+      double QS[NQ];
+      if (k > 0 && k < nr) {
+        const double vk = vr[1];
+        DS = vl_rad_star(vk, dt, sg[0], sg[1], dqs[0], dqs[1], t.Rmed[k - 1], t.Rmed[k], t.Rmed[k + 1], k);
+        for (int q = 0; q < NQ; q++)
+          QS[q] = vl_rad_star(vk, dt, W[q][0], W[q][1], dq[q][0], dq[q][1], t.Rmed[k - 1], t.Rmed[k], t.Rmed[k + 1], k);
+      } else {
+        for (int q = 0; q < NQ; q++) QS[q] = 0.0;
+      }
+      const double vface = (k < nr) ? vr[1] : 0.0;
+      for (int q = 0; q < NQ; q++) {
+        F[q][0] = F[q][1];
+        F[q][1] = cflux * t.Rinf[k] * QS[q] * DS * vface;
+      }
+      // ring k-1 is complete: Q += (F(k-1) - F(k)) / Surf
+      const int o = k - 1;
+      if (o >= b0 && o < b1) {
+        const long lo = (long)o * ns + j;
+        for (int q = 0; q < NQ; q++) {
+          double varq = F[q][0];
+          varq -= F[q][1];
+          a.out[q][lo] = Q[q][0] + varq * t.invSurf[o];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ k_taz ----
+// OneWindTheta (TransportEuler.cu:177-194) + the vtheta half of
+// ComputeVelocities (Kernels.cu:1206-1207) for one ring per workgroup.
+// Thread t owns sectors j = t + k*blockDim (k < C); the five quantities
+// stay in registers, neighbours are exchanged through two LDS rings.
+struct taz_args {
+  const double *vt;    // vtheta after k_src (ring mean and residual)
+  const double *q[NQ];  // quantities after the radial sweep
+  double *xd, *xvt, *xrmp, *xrmm;
+  double *vmed;
+  int *nshift;
+  fg_dev_tables t;
+  double dt, omega_frame;
+  int nr, ns, fast;
+};
+
+template <int C>
+__device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double (&V)[C], double *W, double *D,
+                                                int ns, int i, const fg_dev_tables &t, double dt) {
+  const int BS = blockDim.x, tid = threadIdx.x;
+  const double invdxtheta = t.invdxtheta[i], dxtheta = t.dxtheta[i];
+  const double dxrad = (t.Rsup[i] - t.Rinf[i]) * dt;
+  const double invsurf = t.invSurf[i];
+  double dens[C], DS[C];
+  // DensStar from Sigma (ComputeStarTheta(Dens), TransportEuler.cu:232)
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    dens[k] = q[NQ - 1][k];
+    if (j < ns) W[j] = dens[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    if (j < ns) D[j] = vl_theta_slope(W[(j - 1 + ns) % ns], W[j], W[(j + 1) % ns], invdxtheta);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    if (j < ns) {
+      int jm = (j - 1 + ns) % ns;
+      DS[k] = vl_theta_star(V[k], dt, W[jm], W[j], D[jm], D[j], dxtheta);
+    }
+  }
+  __syncthreads();
+  // VanLeerTheta for each quantity, Sigma last (TransportEuler.cu:234-243)
+#pragma unroll
+  for (int qq = 0; qq < NQ; qq++) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) W[j] = q[qq][k] / (dens[k] + 1e-20);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) D[j] = vl_theta_slope(W[(j - 1 + ns) % ns], W[j], W[(j + 1) % ns], invdxtheta);
+    }
+    __syncthreads();
+    double F[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      F[k] = 0.0;
+      if (j < ns) {
+        int jm = (j - 1 + ns) % ns;
+        double QS = vl_theta_star(V[k], dt, W[jm], W[j], D[jm], D[j], dxtheta);
+        F[k] = dxrad * QS * DS[k] * V[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) W[j] = F[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) {
+        double varq = F[k];
+        varq -= W[(j + 1) % ns];
+        q[qq][k] += varq * invsurf;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(1024) void k_taz(taz_args a) {
+  extern __shared__ double lds[];
+  double *W = lds, *D = lds + a.ns;
+  __shared__ double s_vmed;
+  const int ns = a.ns, i = blockIdx.x, BS = blockDim.x, tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const long base = (long)i * ns;
+  double vt[C], q[NQ][C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    vt[k] = 0.0;
+    for (int qq = 0; qq < NQ; qq++) q[qq][k] = 0.0;
+    if (j < ns) {
+      vt[k] = a.vt[base + j];
+      W[j] = vt[k];
+      for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
+    }
+  }
+  __syncthreads();
+  // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081): the
+  // sequential j sum keeps VMed, and so Nshift, bit-identical
+  if (tid == 0) {
+    double moy = 0.0;
+    for (int j = 0; j < ns; j++) moy += W[j];
+    s_vmed = moy / (double)ns;
+  }
+  __syncthreads();
+  const double VMed = s_vmed;
+  // ComputeConstantResidualKernel (Kernels.cu:1094-1126)
+  const double invdt = 1.0 / a.dt;
+  const double dpinvns = 2.0 * FG_PI / (double)ns;
+  const double Ntilde = VMed * t.invRmed[i] * a.dt * (double)ns / 2.0 / FG_PI;
+  const double Nround = floor(Ntilde + 0.5);
+  const int nsh = (int)(long)Nround;
+  double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
+  double vres[C], vu[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    vres[k] = vt[k] - VMed;
+    vu[k] = vuni;
+    if (!a.fast) {
+      vres[k] += vuni;
+      vu[k] = 0.0;
+    }
+  }
+  if (tid == 0) {
+    a.vmed[i] = VMed;
+    a.nshift[i] = nsh;
+  }
+  azimuthal_sweep<C>(q, vres, W, D, ns, i, t, a.dt);
+  azimuthal_sweep<C>(q, vu, W, D, ns, i, t, a.dt);
+  // AdvectSHIFT (Kernels.cu:1173-1190) of every quantity
+#pragma unroll
+  for (int qq = 0; qq < NQ; qq++) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) W[j] = q[qq][k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      int j = tid + k * BS;
+      if (j < ns) {
+        int ji = (j - nsh) % ns;
+        if (ji < 0) ji += ns;
+        q[qq][k] = W[ji];
+      }
+    }
+    __syncthreads();
+  }
+  // vtheta from the angular momenta (Kernels.cu:1206-1207)
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    if (j < ns) {
+      W[j] = q[2][k];
+      D[j] = q[NQ - 1][k];
+    }
+  }
+  __syncthreads();
+  const double Rm = t.Rmed[i];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    int j = tid + k * BS;
+    if (j < ns) {
+      int jm = (j - 1 + ns) % ns;
+      double s = q[NQ - 1][k];
+      double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
+      a.xd[base + j] = s;
+      a.xvt[base + j] = v;
+      a.xrmp[base + j] = q[0][k];
+      a.xrmm[base + j] = q[1][k];
+    }
+  }
+}
+
+// ------------------------------------------------------------ k_fin ----
+// vrad recovery (Kernels.cu:1200-1205), the third damping pass, the
+// temperature and, for the next substep, ConditionCFLKernel1D/2D1/2D2 of
+// this ring (vrad of ring i+1 is recomputed here, it is needed by
+// Kernels.cu:1302).
+struct fin_args {
+  const double *xd, *xvt, *xrmp, *xrmm, *press;
+  double *dens, *vr, *vt, *temp;
+  double *vmoy, *newdt;
+  const double *nu;
+  fg_dev_tables t;
+  double dt, omega_frame;
+  int nr, ns, evanescent, fast;
+};
+
+__device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
+  const int ns = a.ns;
+  if (i == 0) return 0.0;
+  if (i >= a.nr) return 0.0;
+  long l = (long)i * ns + j;
+  return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
+}
+
+__global__ __launch_bounds__(256) void k_fin(fin_args a) {
+  extern __shared__ double ring[];  // final vtheta of the ring
+  __shared__ double red[256];
+  __shared__ double s_vmoy;
+  const int ns = a.ns, nr = a.nr, i = blockIdx.x, tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const long base = (long)i * ns;
+  double lam_i = 0, vr0_i = 0, vt0_i = 0, lam_n = 0, vr0_n = 0, vt0_n = 0;
+  const bool damp_i = a.evanescent && t.ev_zone[i];
+  const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
+  if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i);
+  if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
+  for (int j = tid; j < ns; j += 256) {
+    long l = base + j;
+    double vr = recover_vr(a, i, j), vt = a.xvt[l], s = a.xd[l];
+    if (damp_i) {
+      vr = damp_toward(vr, lam_i, vr0_i);
+      vt = damp_toward(vt, lam_i, vt0_i);
+      s = damp_toward(s, lam_i, t.SigmaMed[i]);
+    }
+    a.dens[l] = s;
+    a.vr[l] = vr;
+    a.vt[l] = vt;
+    ring[j] = vt;
+    if (a.temp) a.temp[l] = a.press[l] / s;
+    if (i == nr - 1) {  // ghost ring after transport: all zero
+      a.dens[l + ns] = 0.0;
+      a.vr[l + ns] = 0.0;
+      a.vt[l + ns] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int j = 0; j < ns; j++) m += ring[j];
+    m /= (double)ns;
+    a.vmoy[i] = m;
+    s_vmoy = m;
+  }
+  __syncthreads();
+  if (i == 0) return;
+  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
+  const double dxrad = t.Rsup[i] - t.Rinf[i];
+  const double dxtheta = t.dxtheta_cfl[i];
+  const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
+  const double vm = s_vmoy, cs = t.cs_iso[i];
+  double best = 1e30;
+  for (int j = tid; j < ns; j += 256) {
+    long l = base + j;
+    double vr = a.vr[l];  // written above by this thread
+    double vrn = recover_vr(a, i + 1, j);
+    if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+    double vres = a.fast ? (ring[j] - vm) : ring[j];
+    double invdt1 = cs / mindx;
+    double invdt2 = fabs(vr) / dxrad;
+    double invdt3 = fabs(vres) / dxtheta;
+    double dvr = vrn - vr;
+    double dvt = ring[(j + 1) % ns] - ring[j];
+    dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
+    dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
+    double x = dvr / dxrad, y = dvt / dxtheta;
+    double invdt4 = (y > x) ? y : x;
+    invdt4 *= 4.0 * 1.41 * 1.41;
+    double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
+    if (dtc < best) best = dtc;
+  }
+  red[tid] = best;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) a.newdt[i] = red[0];
+}
+
+// ----------------------------------------------------- entry points ----
+static fg_status fused_alloc(fg_ctx *c) {
+  if (c->fu_ring) return FG_OK;
+  const size_t S = (size_t)c->S;
+  // 14 arrays: B (3), R (5), X (4), pressure, spare
+  double *pool = nullptr;
+  FG_HIP(hipMalloc((void **)&pool, 14 * S * sizeof(double)));
+  FG_HIP(hipMemset(pool, 0, 14 * S * sizeof(double)));
+  c->fu_ring = pool;
+  return FG_OK;
+}
+
+static fused_bufs bufs_of(fg_ctx *c) {
+  const size_t S = (size_t)c->S;
+  double *p = c->fu_ring;
+  fused_bufs b;
+  b.Bd = p + 0 * S;
+  b.Bvr = p + 1 * S;
+  b.Bvt = p + 2 * S;
+  b.Rd = p + 3 * S;
+  b.Rrmp = p + 4 * S;
+  b.Rrmm = p + 5 * S;
+  b.Rtmp = p + 6 * S;
+  b.Rtmm = p + 7 * S;
+  b.Xd = p + 8 * S;
+  b.Xvt = p + 9 * S;
+  b.Xrmp = p + 10 * S;
+  b.Xrmm = p + 11 * S;
+  b.P = p + 12 * S;
+  return b;
+}
+
+static fg_status fused_supported(fg_ctx *c) {
+  const fg_params &p = c->p;
+  if (p.adiabatic || p.self_gravity || p.open_inner || p.non_reflecting || !p.evanescent) {
+    fg_set_error("fused path: deck needs the per-routine path (adiabatic, self-gravity or non-evanescent edges)");
+    return FG_E_UNSUPPORTED;
+  }
+  if (c->ns > 4096) {
+    fg_set_error("fused path: NSEC %d > 4096", c->ns);
+    return FG_E_UNSUPPORTED;
+  }
+  return FG_OK;
+}
+
+// results of the reductions for the next substep: CFL count and the disk
+// force on the star; copied to pinned memory behind an event
+static fg_status fused_reductions(fg_ctx *c, double DeltaT) {
+  if (fg_status s = fg_disk_force_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, c->red_out)) return s;
+  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
+                     c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+  FG_LAUNCH_CHECK();
+  FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  return FG_OK;
+}
+
+// CFL ring pass over the current state (first substep, or after the
+// per-routine path touched the state)
+__global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const double *vt, const double *cs,
+                           fg_dev_tables t, int nr, int ns, int fast);
+
 extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
-  (void)c; (void)DeltaT;
-  fg_set_error("fused path not built");
-  return FG_E_UNSUPPORTED;
+  if (!c) return FG_E_INVALID;
+  if (fg_status s = fused_supported(c)) return s;
+  if (fg_status s = fused_alloc(c)) return s;
+  size_t shm = (size_t)(c->ns + 256) * sizeof(double);
+  hipLaunchKernelGGL(k_cfl_ring, dim3(c->nr), dim3(256), shm, c->stream, c->Vmoy, c->newDT, c->f[FG_VRAD],
+                     c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->t, c->nr, c->ns, c->p.fast_transport);
+  FG_LAUNCH_CHECK();
+  return fused_reductions(c, DeltaT);
 }
+
 extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, double *lost) {
-  (void)c; (void)ncfl; (void)force2; (void)lost;
-  fg_set_error("fused path not built");
-  return FG_E_UNSUPPORTED;
+  if (!c) return FG_E_INVALID;
+  FG_HIP(hipStreamSynchronize(c->stream));
+  if (ncfl) *ncfl = c->h_cfl[0];
+  if (force2) {
+    force2[0] = c->h_out[0];
+    force2[1] = c->h_out[1];
+  }
+  if (lost) *lost = 0.0;  // evanescent edges: no outflow is recorded
+  return FG_OK;
+}
+
+static int taz_cells(int ns) { return (ns + 1023) / 1024; }
+
+extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
+                                   const fg_planet_pot *planets, int nb, double ix, double iy,
+                                   double DeltaT_next, int want_temperature) {
+  if (!c) return FG_E_INVALID;
+  if (fg_status s = fused_supported(c)) return s;
+  if (fg_status s = fused_alloc(c)) return s;
+  if (nb < 0 || nb > FG_MAX_PLANETS || (nb > 0 && !planets)) {
+    fg_set_error("fg_step_fused: planet count %d", nb);
+    return FG_E_INVALID;
+  }
+  const int nr = c->nr, ns = c->ns;
+  fused_bufs b = bufs_of(c);
+  // k_src
+  src_args sa;
+  sa.dens = c->f[FG_DENS];
+  sa.vr = c->f[FG_VRAD];
+  sa.vt = c->f[FG_VTHETA];
+  sa.odens = b.Bd;
+  sa.ovr = b.Bvr;
+  sa.ovt = b.Bvt;
+  sa.opress = want_temperature ? b.P : nullptr;
+  sa.nu = c->nu;
+  sa.t = c->t;
+  sa.P.nb = nb;
+  for (int k = 0; k < nb; k++) {
+    sa.P.x[k] = planets[k].x;
+    sa.P.y[k] = planets[k].y;
+    sa.P.m[k] = planets[k].mass;
+    sa.P.smooth[k] = planets[k].smooth;
+    sa.P.invd3[k] = planets[k].inv_d3;
+  }
+  sa.P.ix = ix;
+  sa.P.iy = iy;
+  sa.dt = dt;
+  sa.omega_frame = omega_frame;
+  sa.domega = domega;
+  sa.nr = nr;
+  sa.ns = ns;
+  sa.indirect = c->p.indirect_term;
+  sa.evanescent = c->p.evanescent;
+  sa.corotating = (domega != 0.0) ? 1 : 0;
+  hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
+  FG_LAUNCH_CHECK();
+  // k_trad
+  trad_args ta;
+  ta.dens = b.Bd;
+  ta.vr = b.Bvr;
+  ta.vt = b.Bvt;
+  ta.out[0] = b.Rrmp;
+  ta.out[1] = b.Rrmm;
+  ta.out[2] = b.Rtmp;
+  ta.out[3] = b.Rtmm;
+  ta.out[4] = b.Rd;
+  ta.t = c->t;
+  ta.dt = dt;
+  ta.omega_frame = omega_frame;
+  ta.nr = nr;
+  ta.ns = ns;
+  hipLaunchKernelGGL(k_trad, dim3((ns + TRAD_TPB - 1) / TRAD_TPB, (nr + TRAD_BAND - 1) / TRAD_BAND), dim3(TRAD_TPB), 0,
+                     c->stream, ta);
+  FG_LAUNCH_CHECK();
+  // k_taz
+  taz_args za;
+  za.vt = b.Bvt;
+  za.q[0] = b.Rrmp;
+  za.q[1] = b.Rrmm;
+  za.q[2] = b.Rtmp;
+  za.q[3] = b.Rtmm;
+  za.q[4] = b.Rd;
+  za.xd = b.Xd;
+  za.xvt = b.Xvt;
+  za.xrmp = b.Xrmp;
+  za.xrmm = b.Xrmm;
+  za.vmed = c->VMed;
+  za.nshift = c->Nshift;
+  za.t = c->t;
+  za.dt = dt;
+  za.omega_frame = omega_frame;
+  za.nr = nr;
+  za.ns = ns;
+  za.fast = c->p.fast_transport;
+  const int C = taz_cells(ns);
+  const int bs = ((ns + C - 1) / C + 63) / 64 * 64;
+  const size_t shm = 2 * (size_t)ns * sizeof(double);
+  switch (C) {
+    case 1: hipLaunchKernelGGL(k_taz<1>, dim3(nr), dim3(bs), shm, c->stream, za); break;
+    case 2: hipLaunchKernelGGL(k_taz<2>, dim3(nr), dim3(bs), shm, c->stream, za); break;
+    case 3: hipLaunchKernelGGL(k_taz<3>, dim3(nr), dim3(bs), shm, c->stream, za); break;
+    default: hipLaunchKernelGGL(k_taz<4>, dim3(nr), dim3(bs), shm, c->stream, za); break;
+  }
+  FG_LAUNCH_CHECK();
+  // k_fin
+  fin_args fa;
+  fa.xd = b.Xd;
+  fa.xvt = b.Xvt;
+  fa.xrmp = b.Xrmp;
+  fa.xrmm = b.Xrmm;
+  fa.press = b.P;
+  fa.dens = c->f[FG_DENS];
+  fa.vr = c->f[FG_VRAD];
+  fa.vt = c->f[FG_VTHETA];
+  fa.temp = want_temperature ? c->f[FG_TEMPERATURE] : nullptr;
+  fa.vmoy = c->Vmoy;
+  fa.newdt = c->newDT;
+  fa.nu = c->nu;
+  fa.t = c->t;
+  fa.dt = dt;
+  fa.omega_frame = omega_frame;
+  fa.nr = nr;
+  fa.ns = ns;
+  fa.evanescent = c->p.evanescent;
+  fa.fast = c->p.fast_transport;
+  hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
+  FG_LAUNCH_CHECK();
+  return fused_reductions(c, DeltaT_next);
 }

@@ -290,6 +290,8 @@ struct fg_sim {
   double dt = 0.0, dtemp = 0.0;
   int in_dt = 0, last_cfl = 0;
   double IndirectX = 0.0, IndirectY = 0.0;
+  int fused_active = 0;   // substeps run through fg_step_fused
+  int results_ready = 0;  // fused CFL count / star force are current
 };
 
 namespace {
@@ -910,6 +912,10 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
   s->PhysicalTimeInitial = s->PhysicalTime;
   s->prof.time = s->PhysicalTime;
   s->prof.time0 = s->PhysicalTimeInitial;
+  // the fused path covers locally isothermal decks with evanescent edges
+  // and a per-substep CFL; anything else stays on the per-routine path
+  s->fused_active = s->opt.fused && !d.Adiabatic && !d.SelfGravity && d.Evanescent && !d.OpenInner &&
+                    !d.NonReflecting && !d.SloppyCFL && d.IsDisk && ns <= 4096;
   *out = s;
   return FG_OK;
 }
@@ -921,9 +927,68 @@ extern "C" fg_status fg_sim_destroy(fg_sim *s) {
   return FG_OK;
 }
 
+// AlgoGas's loop body on the fused path: the same host-side sequence, the
+// GPU part in one fg_step_fused call whose reductions already hold the
+// next substep's CFL count and star force.
+static fg_status fused_substep(fg_sim *s, int *closed) {
+  const Deck &d = s->d;
+  fg_ctx *c = s->ctx;
+  if (!s->in_dt) {
+    s->dtemp = 0.0;
+    s->in_dt = 1;
+  }
+  s->MassTaper = s->PhysicalTime / (d.MASSTAPER * 2.0 * M_PI);
+  s->MassTaper = (s->MassTaper > 1.0 ? 1.0 : pow(sin(s->MassTaper * M_PI / 2.0), 2.0));
+  if (!s->results_ready) {
+    CHECK(fg_fused_prepare(c, d.DT - s->dtemp));
+    s->results_ready = 1;
+  }
+  int cfl = 1;
+  double force[2];
+  CHECK(fg_fused_results(c, &cfl, force, nullptr));
+  s->last_cfl = cfl;
+  const double dt = (d.DT - s->dtemp) / (double)cfl;
+  s->dt = dt;
+  s->dtemp += dt;
+  if (d.Corotating) psys_info(s, 1);
+  s->IndirectX = d.Indirect_Term ? -force[0] : 0.0;
+  s->IndirectY = d.Indirect_Term ? -force[1] : 0.0;
+  fg_planet_pot pl[FG_MAX_PLANETS];
+  const int nb = potential_planets(s, pl);
+  Planets &p = s->ps;
+  for (int k = 0; k < p.nb; k++) {  // AdvanceSystemFromDisk, Pframeforce.cu:97-121
+    if (!p.feeldisk[k]) continue;
+    double m = p.mass[k], x = p.x[k], y = p.y[k], r = sqrt(x * x + y * y);
+    double gamma[2];
+    CHECK(fg_disk_force(c, x, y, planet_smoothing(s, r, m, true), gamma));
+    p.vx[k] += dt * gamma[0];
+    p.vy[k] += dt * gamma[1];
+    p.vx[k] += dt * s->IndirectX;
+    p.vy[k] += dt * s->IndirectY;
+  }
+  advance_rk5(s, dt);
+  double domega = 0.0;
+  if (d.Corotating) {
+    double OmegaNew = psys_info(s, 0) / dt;
+    domega = OmegaNew - s->OmegaFrame;
+    s->OmegaFrame = OmegaNew;
+  }
+  rotate_psys(s, s->OmegaFrame * dt);
+  const int done = !(s->dtemp < 0.999999999 * d.DT);
+  const double DeltaT_next = done ? d.DT : d.DT - s->dtemp;
+  CHECK(fg_step_fused(c, dt, s->OmegaFrame, domega, pl, nb, s->IndirectX, s->IndirectY, DeltaT_next,
+                      d.Write_Temperature));
+  s->PhysicalTime += dt;
+  s->prof.time = s->PhysicalTime;
+  if (done) s->in_dt = 0;
+  if (closed) *closed = done;
+  return FG_OK;
+}
+
 // one iteration of the while loop of AlgoGas, SourceEuler.cu:222-305
 extern "C" fg_status fg_sim_substep(fg_sim *s, int *closed) {
   if (!s) return FG_E_INVALID;
+  if (s->fused_active) return fused_substep(s, closed);
   const Deck &d = s->d;
   fg_ctx *c = s->ctx;
   if (!s->in_dt) CHECK(begin_dt(s));
@@ -992,6 +1057,12 @@ extern "C" fg_status fg_sim_substep(fg_sim *s, int *closed) {
 
 extern "C" fg_status fg_sim_algogas(fg_sim *s) {
   if (!s) return FG_E_INVALID;
+  if (s->fused_active) {
+    int closed = 0;
+    do CHECK(fused_substep(s, &closed));
+    while (!closed);
+    return FG_OK;
+  }
   CHECK(begin_dt(s));
   while (s->dtemp < 0.999999999 * s->d.DT) CHECK(fg_sim_substep(s, nullptr));
   s->in_dt = 0;
@@ -1064,6 +1135,7 @@ extern "C" fg_status fg_sim_get_state(fg_sim *s, double *dens, double *vrad, dou
 extern "C" fg_status fg_sim_set_state(fg_sim *s, const double *dens, const double *vrad, const double *vtheta,
                                       const double *energy) {
   if (!s) return FG_E_INVALID;
+  s->results_ready = 0;
   return fg_upload_state(s->ctx, dens, vrad, vtheta, energy);
 }
 
@@ -1080,7 +1152,8 @@ extern "C" fg_status fg_sim_get(fg_sim *s, const char *name, double *v) {
              {"NSEC", (double)s->ns},           {"last_cfl", (double)s->last_cfl},
              {"in_dt", (double)s->in_dt},       {"IndirectTerm.x", s->IndirectX},
              {"IndirectTerm.y", s->IndirectY},  {"HillRadius", s->HillRadius},
-             {"Adiabatic", (double)s->d.Adiabatic}, {"SelfGravity", (double)s->d.SelfGravity}};
+             {"Adiabatic", (double)s->d.Adiabatic}, {"SelfGravity", (double)s->d.SelfGravity},
+             {"fused_active", (double)s->fused_active}};
   for (auto &e : tab)
     if (strcmp(e.k, name) == 0) {
       *v = e.val;

@@ -76,7 +76,8 @@ SIGNATURES = {
     "fg_substep3": (ctypes.c_int, [_vp, _d]),
     "fg_transport": (ctypes.c_int, [_vp, _d, _d, _pd]),
     "fg_selfgravity": (ctypes.c_int, [_vp, _d, ctypes.c_int]),
-    "fg_step_fused": (ctypes.c_int, [_vp, _d, _d, _d, ctypes.POINTER(FgPlanetPot), ctypes.c_int, _d, _d]),
+    "fg_step_fused": (ctypes.c_int, [_vp, _d, _d, _d, ctypes.POINTER(FgPlanetPot), ctypes.c_int, _d, _d, _d,
+                                     ctypes.c_int]),
     "fg_fused_prepare": (ctypes.c_int, [_vp, _d]),
     "fg_fused_results": (ctypes.c_int, [_vp, _pi, _pd, _pd]),
     "fg_sim_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FgSimOptions), ctypes.POINTER(_vp)]),

@@ -192,13 +192,16 @@ fg_status fg_selfgravity(fg_ctx *ctx, double dt, int update);
 
 /* ---- fused substep (same arithmetic, fewer passes over HBM) ---------- */
 /* Everything AlgoGas does on the GPU for one substep after the planets
- * are advanced (SourceEuler.cu:255-295): CorrectVtheta, the three
- * boundary passes, pressure, Substep1/2/3, transport and temperature.
- * Also leaves the CFL sub-cycle count for DeltaT_next (the next substep's
- * ConditionCFL, SourceEuler.cu:228) and the disk force on the star for it
- * (SourceEuler.cu:241), fetched by fg_fused_results. */
+ * are advanced (SourceEuler.cu:255-295): CorrectVtheta (domega, 0 when
+ * the frame does not co-rotate), the potential (FillForcesArrays), the
+ * three boundary passes, pressure, Substep1/2, transport and, when
+ * want_temperature, the temperature.  Also leaves the CFL sub-cycle count
+ * for DeltaT_next (the next substep's ConditionCFL, SourceEuler.cu:228)
+ * and the disk force on the star for it (SourceEuler.cu:241), fetched by
+ * fg_fused_results.  Locally isothermal decks with evanescent edges. */
 fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domega,
-                        const fg_planet_pot *planets, int nb, double indirect_x, double indirect_y);
+                        const fg_planet_pot *planets, int nb, double indirect_x, double indirect_y,
+                        double DeltaT_next, int want_temperature);
 /* CFL count and star force of the current state (fused reductions) */
 fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
 fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);
