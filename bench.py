@@ -1,0 +1,168 @@
+"""Benchmark of the FARGO substep on one MI355X.
+
+A step is one substep (one iteration of AlgoGas's CFL sub-cycle,
+SourceEuler.cu:222) over the whole grid.  Workload: the precision-mode
+grid, 1024 x 3072 (r in [0.4, 2.5], log spacing) with the plain-FARGO
+physics (isothermal h = 0.05, nu = 1e-5, one q = 1e-3 planet, evanescent
+edges) and a seeded 1e-4 white-noise perturbation of Sigma.  Inputs are
+resident in HBM when the timed region starts.
+
+Prints one JSON line: zone-updates/s of the GPU path, the roofline of the
+dominant kernel (HIP-event timing inside the library) and the oracle's
+multicore FP64 rate on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+METRIC = "zone-updates/s + % HBM peak; ensemble time-to-steady-gap, emulator vs plain"
+KERNELS = ["k_src", "k_trad", "k_taz", "k_fin", "force_reduce", "cfl_count"]
+
+
+def algorithmic_bytes(nr, ns, word):
+    """Compulsory HBM bytes of one launch of each fused kernel: every
+    input field read once and every output field written once, over the
+    (NRAD+1) x NSEC field."""
+    S = (nr + 1) * ns
+    return {
+        "k_src": S * word * (3 + 3),   # Sigma, vrad, vtheta in and out
+        "k_trad": S * word * (3 + 5),  # state in; Sigma + 4 momenta out
+        "k_taz": S * word * (6 + 4),   # 5 quantities + vtheta in; 4 out
+        "k_fin": S * word * (4 + 3),   # 4 in; final state out
+        "force_reduce": nr * ns * word,
+        "cfl_count": nr * 8 * 2,
+    }
+
+
+def write_workload_deck(tmp, nr, ns):
+    from conftest import write_deck
+    return write_deck(os.path.join(tmp, "workload.par"), Nrad=nr, Nsec=ns, OutputDir=tmp + "/")
+
+
+def cpu_baseline(deck, nr, ns, steps, warm):
+    """Oracle (FP64, OpenMP per-cell loops, same arithmetic) on the same
+    noisy initial state."""
+    import numpy as np
+    from oracle_binding import Oracle
+    orc = Oracle(deck, omp=True)
+    dens = orc.view("Dens")
+    sig = orc.field("SigmaMed")
+    idx = (np.arange(nr, dtype=np.uint64)[:, None] * np.uint64(ns) + np.arange(ns, dtype=np.uint64)[None, :])
+    x = np.uint64(SEED) ^ idx
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    xi = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+    dens[:nr] = sig[:nr, None] * (1.0 + 1e-4 * xi)
+    for _ in range(warm):
+        orc.substep()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.substep()
+    t1 = time.perf_counter()
+    orc.close()
+    return nr * ns * steps / (t1 - t0), t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nrad", type=int, default=1024)
+    ap.add_argument("--nsec", type=int, default=3072)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--path", choices=["fused", "routine"], default="fused")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(every GPU box here has one GPU)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from cudaprotoplanetarydisks_amd import Simulation, lib
+    from cudaprotoplanetarydisks_amd.native import call
+
+    nr, ns = args.nrad, args.nsec
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = write_workload_deck(tmp, nr, ns)
+        sim = Simulation(deck, fused=(args.path == "fused"), precision=args.precision,
+                         noise_seed=SEED, noise_amp=1e-4)
+        ctx = sim.ctx.h
+        for _ in range(args.warmup):
+            sim.substep()
+        call("fg_sync", ctx)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sim.substep()
+        call("fg_sync", ctx)
+        t1 = time.perf_counter()
+        elapsed = t1 - t0
+        # kernel timing (separate pass so events do not perturb the clock)
+        prof = {}
+        if args.path == "fused":
+            call("fg_profile", ctx, 1)
+            for _ in range(min(args.steps, 50)):
+                sim.substep()
+            ms = (ctypes.c_double * 6)()
+            calls = (ctypes.c_long * 6)()
+            n = lib().fg_profile_read(ctx, ms, calls, 6)
+            call("fg_profile", ctx, 0)
+            prof = {KERNELS[k]: ms[k] / max(1, calls[k]) for k in range(n)}
+        d, _, _ = sim.state()
+        finite = bool(np.isfinite(d).all())
+        sim.close()
+        word = 8 if args.precision == "fp64" else 4
+        zus = nr * ns * args.steps / elapsed
+        b_alg = 48 if word == 8 else 24
+        roofline = None
+        if prof:
+            dom = max(prof, key=prof.get)
+            ab = algorithmic_bytes(nr, ns, word)[dom]
+            achieved = ab / (prof[dom] * 1e-3) / 1e9
+            traffic = None
+            tfile = os.path.join(ROOT, "profiles", "traffic.json")
+            if os.path.exists(tfile):
+                traffic = json.load(open(tfile)).get(f"{args.precision}:{dom}")
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+        cpu = None
+        if not args.no_cpu:
+            threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            rate, secs = cpu_baseline(deck, nr, ns, args.cpu_steps, 1)
+            cpu = {"value": round(rate, 1), "unit": "zone-updates/s", "cores": threads, "kind": "port",
+                   "sample": f"oracle FP64 (OpenMP), {nr}x{ns}, 1 warm-up + {args.cpu_steps} substeps "
+                             f"({secs:.1f} s)"}
+    out = {
+        "metric": METRIC, "value": round(zus, 1), "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if word == 8 else "f32", "data": "synthetic",
+        "config": {"workload": f"plain FARGO substep, {nr}x{ns} log grid, isothermal h=0.05, nu=1e-5, "
+                               "q=1e-3 planet, evanescent edges, seeded 1e-4 noise on Sigma",
+                   "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision},
+        "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
+        "roofline": roofline, "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
+        "finite": finite,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -284,17 +284,48 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
 struct trad_args {
   const double *dens, *vr, *vt;
   double *out[NQ];
+  double *vmed;  // ring means of vtheta for the azimuthal pass
   fg_dev_tables t;
   double dt, omega_frame;
-  int nr, ns;
+  int nr, ns, nstrips;
 };
+
+// ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081) for one ring
+// by one wave: lanes stream the ring through LDS 64 sectors at a time
+// (next chunk prefetched into registers) and lane 0 adds them in j order.
+__device__ void ring_mean_wave(const double *v, int ns, double *out) {
+  __shared__ double buf[2][64];
+  const int lane = threadIdx.x;
+  const int nch = (ns + 63) / 64;
+  double next = (lane < ns) ? v[lane] : 0.0;
+  double sum = 0.0;
+  for (int c = 0; c < nch; c++) {
+    double cur = next;
+    int e = (c + 1) * 64 + lane;
+    next = (c + 1 < nch && e < ns) ? v[e] : 0.0;
+    buf[c & 1][lane] = cur;
+    __syncthreads();
+    if (lane == 0) {
+      int n = min(64, ns - c * 64);
+      for (int k = 0; k < n; k++) sum += buf[c & 1][k];
+    }
+  }
+  if (lane == 0) *out = sum / (double)ns;
+}
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   const int nr = a.nr, ns = a.ns;
-  const int j = blockIdx.x * TRAD_TPB + threadIdx.x;
+  // blocks [0, NRAD): one ring mean each, issued first so that these
+  // latency-bound sums overlap the sweep blocks
+  if ((int)blockIdx.x < nr) {
+    ring_mean_wave(a.vt + (long)blockIdx.x * ns, ns, a.vmed + blockIdx.x);
+    return;
+  }
+  const int bid = blockIdx.x - nr;
+  const int j = (bid % a.nstrips) * TRAD_TPB + threadIdx.x;
   if (j >= ns) return;
   const int jp = (j + 1) % ns;
-  const int b0 = blockIdx.y * TRAD_BAND;
+  const int b0 = (bid / a.nstrips) * TRAD_BAND;
   const int b1 = min(b0 + TRAD_BAND, nr);
   const fg_dev_tables &t = a.t;
   const double dt = a.dt, OF = a.omega_frame;
@@ -374,127 +405,89 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
 
 // ------------------------------------------------------------ k_taz ----
 // OneWindTheta (TransportEuler.cu:177-194) + the vtheta half of
-// ComputeVelocities (Kernels.cu:1206-1207) for one ring per workgroup.
-// Thread t owns sectors j = t + k*blockDim (k < C); the five quantities
-// stay in registers, neighbours are exchanged through two LDS rings.
+// ComputeVelocities (Kernels.cu:1206-1207) for one ring per 256-thread
+// workgroup.  Thread t owns sectors j = t + 256 k (k < C); the five
+// quantities stay in registers, neighbours are exchanged through two LDS
+// rings.  VMed comes from the ring-mean waves of k_trad.
+#define TAZ_TPB 256
+
 struct taz_args {
-  const double *vt;    // vtheta after k_src (ring mean and residual)
+  const double *vt;     // vtheta after k_src
+  const double *vmed;   // its ring means
   const double *q[NQ];  // quantities after the radial sweep
   double *xd, *xvt, *xrmp, *xrmm;
-  double *vmed;
   int *nshift;
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, fast;
 };
 
-template <int C>
-__device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double (&V)[C], double *W, double *D,
-                                                int ns, int i, const fg_dev_tables &t, double dt) {
-  const int BS = blockDim.x, tid = threadIdx.x;
+#define FOR_CELLS(k, j) \
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+
+// QuantitiesAdvection (TransportEuler.cu:229-245) with velocity V (one
+// value per cell, or the ring's uniform residual when UNIFORM)
+template <int C, bool UNIFORM>
+__device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double (&Vc)[C], double Vu, double *W,
+                                                double *D, int ns, int i, const fg_dev_tables &t, double dt) {
   const double invdxtheta = t.invdxtheta[i], dxtheta = t.dxtheta[i];
   const double dxrad = (t.Rsup[i] - t.Rinf[i]) * dt;
   const double invsurf = t.invSurf[i];
   double dens[C], DS[C];
+#define V(k) (UNIFORM ? Vu : Vc[k])
   // DensStar from Sigma (ComputeStarTheta(Dens), TransportEuler.cu:232)
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
+  FOR_CELLS(k, j) {
     dens[k] = q[NQ - 1][k];
-    if (j < ns) W[j] = dens[k];
+    W[j] = dens[k];
   }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
-    if (j < ns) D[j] = vl_theta_slope(W[(j - 1 + ns) % ns], W[j], W[(j + 1) % ns], invdxtheta);
+  FOR_CELLS(k, j) D[j] = vl_theta_slope(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
+  __syncthreads();
+  FOR_CELLS(k, j) {
+    int jm = j == 0 ? ns - 1 : j - 1;
+    DS[k] = vl_theta_star(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta);
   }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
-    if (j < ns) {
-      int jm = (j - 1 + ns) % ns;
-      DS[k] = vl_theta_star(V[k], dt, W[jm], W[j], D[jm], D[j], dxtheta);
-    }
-  }
-  __syncthreads();
-  // VanLeerTheta for each quantity, Sigma last (TransportEuler.cu:234-243)
+  // VanLeerTheta of each quantity, Sigma last (TransportEuler.cu:234-243)
 #pragma unroll
   for (int qq = 0; qq < NQ; qq++) {
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) W[j] = q[qq][k] / (dens[k] + 1e-20);
-    }
+    FOR_CELLS(k, j) W[j] = q[qq][k] / (dens[k] + 1e-20);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) D[j] = vl_theta_slope(W[(j - 1 + ns) % ns], W[j], W[(j + 1) % ns], invdxtheta);
-    }
+    FOR_CELLS(k, j) D[j] = vl_theta_slope(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
     __syncthreads();
     double F[C];
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      F[k] = 0.0;
-      if (j < ns) {
-        int jm = (j - 1 + ns) % ns;
-        double QS = vl_theta_star(V[k], dt, W[jm], W[j], D[jm], D[j], dxtheta);
-        F[k] = dxrad * QS * DS[k] * V[k];
-      }
+    FOR_CELLS(k, j) {
+      int jm = j == 0 ? ns - 1 : j - 1;
+      double QS = vl_theta_star(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta);
+      F[k] = dxrad * QS * DS[k] * V(k);
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) W[j] = F[k];
-    }
+    FOR_CELLS(k, j) W[j] = F[k];
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) {
-        double varq = F[k];
-        varq -= W[(j + 1) % ns];
-        q[qq][k] += varq * invsurf;
-      }
+    FOR_CELLS(k, j) {
+      double varq = F[k];
+      varq -= W[j + 1 == ns ? 0 : j + 1];
+      q[qq][k] += varq * invsurf;
     }
     __syncthreads();
   }
+#undef V
 }
 
 template <int C>
-__global__ __launch_bounds__(1024) void k_taz(taz_args a) {
+__global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
   extern __shared__ double lds[];
   double *W = lds, *D = lds + a.ns;
-  __shared__ double s_vmed;
-  const int ns = a.ns, i = blockIdx.x, BS = blockDim.x, tid = threadIdx.x;
+  const int ns = a.ns, i = blockIdx.x, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
-  double vt[C], q[NQ][C];
+  const double VMed = a.vmed[i];
+  double q[NQ][C], vres[C];
+  FOR_CELLS(k, j) {
+    vres[k] = a.vt[base + j] - VMed;  // ComputeResidualsKernel
 #pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
-    vt[k] = 0.0;
-    for (int qq = 0; qq < NQ; qq++) q[qq][k] = 0.0;
-    if (j < ns) {
-      vt[k] = a.vt[base + j];
-      W[j] = vt[k];
-      for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
-    }
+    for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
   }
-  __syncthreads();
-  // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081): the
-  // sequential j sum keeps VMed, and so Nshift, bit-identical
-  if (tid == 0) {
-    double moy = 0.0;
-    for (int j = 0; j < ns; j++) moy += W[j];
-    s_vmed = moy / (double)ns;
-  }
-  __syncthreads();
-  const double VMed = s_vmed;
   // ComputeConstantResidualKernel (Kernels.cu:1094-1126)
   const double invdt = 1.0 / a.dt;
   const double dpinvns = 2.0 * FG_PI / (double)ns;
@@ -502,67 +495,44 @@ __global__ __launch_bounds__(1024) void k_taz(taz_args a) {
   const double Nround = floor(Ntilde + 0.5);
   const int nsh = (int)(long)Nround;
   double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
-  double vres[C], vu[C];
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    vres[k] = vt[k] - VMed;
-    vu[k] = vuni;
-    if (!a.fast) {
-      vres[k] += vuni;
-      vu[k] = 0.0;
-    }
+  if (!a.fast) {
+    FOR_CELLS(k, j) vres[k] += vuni;
+    vuni = 0.0;
   }
-  if (tid == 0) {
-    a.vmed[i] = VMed;
-    a.nshift[i] = nsh;
-  }
-  azimuthal_sweep<C>(q, vres, W, D, ns, i, t, a.dt);
-  azimuthal_sweep<C>(q, vu, W, D, ns, i, t, a.dt);
+  if (tid == 0) a.nshift[i] = nsh;
+  azimuthal_sweep<C, false>(q, vres, 0.0, W, D, ns, i, t, a.dt);
+  azimuthal_sweep<C, true>(q, vres, vuni, W, D, ns, i, t, a.dt);
   // AdvectSHIFT (Kernels.cu:1173-1190) of every quantity
+  int sh = nsh % ns;
+  if (sh < 0) sh += ns;
 #pragma unroll
   for (int qq = 0; qq < NQ; qq++) {
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) W[j] = q[qq][k];
-    }
+    FOR_CELLS(k, j) W[j] = q[qq][k];
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      int j = tid + k * BS;
-      if (j < ns) {
-        int ji = (j - nsh) % ns;
-        if (ji < 0) ji += ns;
-        q[qq][k] = W[ji];
-      }
+    FOR_CELLS(k, j) {
+      int ji = j - sh;
+      q[qq][k] = W[ji < 0 ? ji + ns : ji];
     }
     __syncthreads();
   }
   // vtheta from the angular momenta (Kernels.cu:1206-1207)
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
-    if (j < ns) {
-      W[j] = q[2][k];
-      D[j] = q[NQ - 1][k];
-    }
+  FOR_CELLS(k, j) {
+    W[j] = q[2][k];
+    D[j] = q[NQ - 1][k];
   }
   __syncthreads();
   const double Rm = t.Rmed[i];
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    int j = tid + k * BS;
-    if (j < ns) {
-      int jm = (j - 1 + ns) % ns;
-      double s = q[NQ - 1][k];
-      double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
-      a.xd[base + j] = s;
-      a.xvt[base + j] = v;
-      a.xrmp[base + j] = q[0][k];
-      a.xrmm[base + j] = q[1][k];
-    }
+  FOR_CELLS(k, j) {
+    int jm = j == 0 ? ns - 1 : j - 1;
+    double s = q[NQ - 1][k];
+    double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
+    a.xd[base + j] = s;
+    a.xvt[base + j] = v;
+    a.xrmp[base + j] = q[0][k];
+    a.xrmm[base + j] = q[1][k];
   }
 }
+#undef FOR_CELLS
 
 // ------------------------------------------------------------ k_fin ----
 // vrad recovery (Kernels.cu:1200-1205), the third damping pass, the
@@ -663,6 +633,52 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
 }
 
 // ----------------------------------------------------- entry points ----
+#define PROF_KERNELS 6
+
+static void prof_mark(fg_ctx *c) {
+  if (c->prof_on && c->prof_nev < 16) (void)hipEventRecord(c->prof_ev[c->prof_nev++], c->stream);
+}
+
+// fold a completed substep's events into the totals (stream is idle)
+static void prof_fold(fg_ctx *c) {
+  if (!c->prof_on || !c->prof_pending) return;
+  for (int k = 0; k + 1 < c->prof_nev && k < PROF_KERNELS; k++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]) == hipSuccess) {
+      c->prof_ms[k] += ms;
+      c->prof_calls[k]++;
+    }
+  }
+  c->prof_pending = 0;
+  c->prof_nev = 0;
+}
+
+extern "C" fg_status fg_profile(fg_ctx *c, int enable) {
+  if (!c) return FG_E_INVALID;
+  FG_HIP(hipStreamSynchronize(c->stream));
+  if (enable && !c->prof_ev[0])
+    for (int k = 0; k < 16; k++) FG_HIP(hipEventCreate(&c->prof_ev[k]));
+  for (int k = 0; k < 16; k++) {
+    c->prof_ms[k] = 0.0;
+    c->prof_calls[k] = 0;
+  }
+  c->prof_nev = 0;
+  c->prof_pending = 0;
+  c->prof_on = enable;
+  return FG_OK;
+}
+
+extern "C" int fg_profile_read(fg_ctx *c, double *ms, long *calls, int max) {
+  if (!c) return 0;
+  (void)hipStreamSynchronize(c->stream);
+  prof_fold(c);
+  int n = max < PROF_KERNELS ? max : PROF_KERNELS;
+  for (int k = 0; k < n; k++) {
+    if (ms) ms[k] = c->prof_ms[k];
+    if (calls) calls[k] = c->prof_calls[k];
+  }
+  return n;
+}
 static fg_status fused_alloc(fg_ctx *c) {
   if (c->fu_ring) return FG_OK;
   const size_t S = (size_t)c->S;
@@ -711,9 +727,11 @@ static fg_status fused_supported(fg_ctx *c) {
 // force on the star; copied to pinned memory behind an event
 static fg_status fused_reductions(fg_ctx *c, double DeltaT) {
   if (fg_status s = fg_disk_force_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, c->red_out)) return s;
+  prof_mark(c);
   hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
                      c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
   FG_LAUNCH_CHECK();
+  prof_mark(c);
   FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   return FG_OK;
@@ -738,6 +756,7 @@ extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
 extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, double *lost) {
   if (!c) return FG_E_INVALID;
   FG_HIP(hipStreamSynchronize(c->stream));
+  prof_fold(c);
   if (ncfl) *ncfl = c->h_cfl[0];
   if (force2) {
     force2[0] = c->h_out[0];
@@ -747,7 +766,13 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   return FG_OK;
 }
 
-static int taz_cells(int ns) { return (ns + 1023) / 1024; }
+// cells per thread of k_taz: the smallest instantiated C with 256 C >= NSEC
+static int taz_cells(int ns) {
+  const int cs[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  for (int c : cs)
+    if (TAZ_TPB * c >= ns) return c;
+  return 0;
+}
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
@@ -790,6 +815,13 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   sa.indirect = c->p.indirect_term;
   sa.evanescent = c->p.evanescent;
   sa.corotating = (domega != 0.0) ? 1 : 0;
+  if (c->prof_on) {
+    (void)hipStreamSynchronize(c->stream);
+    prof_fold(c);
+    c->prof_nev = 0;
+    c->prof_pending = 1;
+  }
+  prof_mark(c);
   hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
   FG_LAUNCH_CHECK();
   // k_trad
@@ -807,12 +839,16 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.omega_frame = omega_frame;
   ta.nr = nr;
   ta.ns = ns;
-  hipLaunchKernelGGL(k_trad, dim3((ns + TRAD_TPB - 1) / TRAD_TPB, (nr + TRAD_BAND - 1) / TRAD_BAND), dim3(TRAD_TPB), 0,
+  ta.vmed = c->VMed;
+  ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
+  prof_mark(c);
+  hipLaunchKernelGGL(k_trad, dim3(nr + ta.nstrips * ((nr + TRAD_BAND - 1) / TRAD_BAND)), dim3(TRAD_TPB), 0,
                      c->stream, ta);
   FG_LAUNCH_CHECK();
   // k_taz
   taz_args za;
   za.vt = b.Bvt;
+  za.vmed = c->VMed;
   za.q[0] = b.Rrmp;
   za.q[1] = b.Rrmm;
   za.q[2] = b.Rtmp;
@@ -822,7 +858,6 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.xvt = b.Xvt;
   za.xrmp = b.Xrmp;
   za.xrmm = b.Xrmm;
-  za.vmed = c->VMed;
   za.nshift = c->Nshift;
   za.t = c->t;
   za.dt = dt;
@@ -830,14 +865,18 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.nr = nr;
   za.ns = ns;
   za.fast = c->p.fast_transport;
-  const int C = taz_cells(ns);
-  const int bs = ((ns + C - 1) / C + 63) / 64 * 64;
+  prof_mark(c);
   const size_t shm = 2 * (size_t)ns * sizeof(double);
-  switch (C) {
-    case 1: hipLaunchKernelGGL(k_taz<1>, dim3(nr), dim3(bs), shm, c->stream, za); break;
-    case 2: hipLaunchKernelGGL(k_taz<2>, dim3(nr), dim3(bs), shm, c->stream, za); break;
-    case 3: hipLaunchKernelGGL(k_taz<3>, dim3(nr), dim3(bs), shm, c->stream, za); break;
-    default: hipLaunchKernelGGL(k_taz<4>, dim3(nr), dim3(bs), shm, c->stream, za); break;
+  const dim3 g(nr), blk(TAZ_TPB);
+  switch (taz_cells(ns)) {
+    case 1: hipLaunchKernelGGL(k_taz<1>, g, blk, shm, c->stream, za); break;
+    case 2: hipLaunchKernelGGL(k_taz<2>, g, blk, shm, c->stream, za); break;
+    case 3: hipLaunchKernelGGL(k_taz<3>, g, blk, shm, c->stream, za); break;
+    case 4: hipLaunchKernelGGL(k_taz<4>, g, blk, shm, c->stream, za); break;
+    case 6: hipLaunchKernelGGL(k_taz<6>, g, blk, shm, c->stream, za); break;
+    case 8: hipLaunchKernelGGL(k_taz<8>, g, blk, shm, c->stream, za); break;
+    case 12: hipLaunchKernelGGL(k_taz<12>, g, blk, shm, c->stream, za); break;
+    default: hipLaunchKernelGGL(k_taz<16>, g, blk, shm, c->stream, za); break;
   }
   FG_LAUNCH_CHECK();
   // k_fin
@@ -861,7 +900,9 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.ns = ns;
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
+  prof_mark(c);
   hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   FG_LAUNCH_CHECK();
+  prof_mark(c);
   return fused_reductions(c, DeltaT_next);
 }

@@ -87,6 +87,13 @@ struct fg_ctx {
   // host copies of ring values the host-side parts of a routine need
   std::vector<double> h_Rmed, h_Rinf, h_SigmaMed, h_EnergyMed, h_vtheta_arg;
   double lr01, lr12;  // log(Rmed0/Rmed1), log(Rmed1/Rmed2), Kernels.cu:93-94
+  // kernel timing of the fused path (fg_profile): events around each
+  // launch of one substep, folded into per-kernel totals once complete
+  int prof_on;
+  hipEvent_t prof_ev[16];
+  int prof_nev, prof_pending;
+  double prof_ms[16];
+  long prof_calls[16];
   // fused-path scratch
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results

@@ -94,10 +94,15 @@ __global__ __launch_bounds__(RED_TPB) void k_dreduce_partial(Src src, long n, in
 // result is fx_inner + fx_outer of SideEuler.cu:144 where the inner sum
 // is the DeviceReduce of an all-zero array (+0.0).
 __global__ void k_dreduce_final(const double *partial, int B, double *out, int two, int add_inner) {
+  __shared__ double p[2 * FG_RED_BLOCKS];
+  const int nv = two ? 2 : 1;
+  // stage the block sums in LDS with every lane, then add them in order
+  for (int k = threadIdx.x; k < nv * B; k += blockDim.x) p[k] = partial[(k / B) * B + (k % B)];
+  __syncthreads();
   int lane = threadIdx.x;
-  if (lane >= (two ? 2 : 1)) return;
+  if (lane >= nv) return;
   double s = 0.0;
-  for (int b = 0; b < B; b++) s += partial[lane * B + b];
+  for (int b = 0; b < B; b++) s += p[lane * B + b];
   if (add_inner) s = 0.0 + s;
   out[lane] = s;
 }
@@ -107,7 +112,7 @@ static fg_status dreduce(fg_ctx *c, Src src, long n, double *out_dev, int two, i
   int B = red_blocks(n);
   hipLaunchKernelGGL(k_dreduce_partial<Src>, dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B, c->red_partial);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dreduce_final, dim3(1), dim3(64), 0, c->stream, c->red_partial, B, out_dev, two, add_inner);
+  hipLaunchKernelGGL(k_dreduce_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, out_dev, two, add_inner);
   FG_LAUNCH_CHECK();
   return FG_OK;
 }

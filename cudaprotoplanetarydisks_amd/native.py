@@ -80,6 +80,8 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "fg_fused_prepare": (ctypes.c_int, [_vp, _d]),
     "fg_fused_results": (ctypes.c_int, [_vp, _pi, _pd, _pd]),
+    "fg_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fg_profile_read": (ctypes.c_int, [_vp, _pd, ctypes.POINTER(ctypes.c_long), ctypes.c_int]),
     "fg_sim_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FgSimOptions), ctypes.POINTER(_vp)]),
     "fg_sim_destroy": (ctypes.c_int, [_vp]),
     "fg_sim_substep": (ctypes.c_int, [_vp, _pi]),

@@ -206,6 +206,17 @@ fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domeg
 fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
 fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);
 
+/* ---- tracing ---------------------------------------------------------- */
+/* Kernel timing of the fused path with HIP events on the context's
+ * stream: enable = 1 starts (and clears) it, 0 stops it.
+ * fg_profile_read fills ms[k] (total milliseconds) and calls[k] for the
+ * kernels k = 0 k_src, 1 k_trad, 2 k_taz, 3 k_fin, 4 disk-force reduction,
+ * 5 CFL count; returns the number of kernels.  No reference counterpart
+ * (the reference only prints the wall time of the whole loop,
+ * Main.cu:308-337). */
+fg_status fg_profile(fg_ctx *ctx, int enable);
+int fg_profile_read(fg_ctx *ctx, double *ms, long *calls, int max);
+
 /* ---- host driver: the reference's main()/AlgoGas over this library ---- */
 typedef struct fg_sim fg_sim;
 typedef struct fg_sim_options {
