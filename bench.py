@@ -45,7 +45,7 @@ def algorithmic_bytes(nr, ns, word):
 
 
 def write_workload_deck(tmp, nr, ns):
-    from conftest import write_deck
+    from cudaprotoplanetarydisks_amd.workloads import write_deck
     return write_deck(os.path.join(tmp, "workload.par"), Nrad=nr, Nsec=ns, OutputDir=tmp + "/")
 
 

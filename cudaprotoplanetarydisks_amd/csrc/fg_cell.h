@@ -50,3 +50,42 @@ FG_DEV double art_visc_q(double dens, double dv) {
 
 // evanescent relaxation, Kernels.cu:1006-1012
 FG_DEV double damp_toward(double q, double lambda, double q0) { return (q + lambda * q0) / (1.0 + lambda); }
+
+// ---- FP32 arithmetic of the bulk-precision mode ----------------------
+// Same formulas as above in single precision.  Quotients use the hardware
+// reciprocal (1 ulp): the FP32 mode is held to an accuracy bound against
+// the FP64 solution, not to bit equality.
+FG_DEV float rcp_f(float b) { return __builtin_amdgcn_rcpf(b); }
+
+FG_DEV float vl_rad_slope_f(float qm, float q0, float qp, float idrm_i, float idrm_ip) {
+  float dqm = (q0 - qm) * idrm_i;
+  float dqp = (qp - q0) * idrm_ip;
+  if (dqp * dqm > 0.0f) return 2.0f * dqp * dqm * rcp_f(dqp + dqm);
+  return 0.0f;
+}
+
+// dm = Rmed[i] - Rmed[i-1], dp = Rmed[i+1] - Rmed[i], formed in FP64 by
+// the caller (the ring spacing is ~1e-3 of the radius)
+FG_DEV float vl_rad_star_f(float vr, float dt, float q_im1, float q_i, float dq_im1, float dq_i, float dm,
+                           float dp) {
+  if (vr > 0.0f) return q_im1 + (dm - vr * dt) * 0.5f * dq_im1;
+  return q_i - (dp + vr * dt) * 0.5f * dq_i;
+}
+
+FG_DEV float vl_theta_slope_f(float qm, float q0, float qp, float invdxtheta) {
+  float dqm = (q0 - qm);
+  float dqp = (qp - q0);
+  if (dqp * dqm > 0.0f) return dqp * dqm * rcp_f(dqp + dqm) * invdxtheta;
+  return 0.0f;
+}
+
+FG_DEV float vl_theta_star_f(float v, float dt, float q_jm1, float q_j, float dq_jm1, float dq_j, float dxtheta) {
+  float ksi = v * dt;
+  if (ksi > 0.0f) return q_jm1 + (dxtheta - ksi) * dq_jm1;
+  return q_j - (dxtheta + ksi) * dq_j;
+}
+
+FG_DEV float art_visc_q_f(float dens, float dv) {
+  if (dv < 0.0f) return (1.41f * 1.41f) * dens * dv * dv;
+  return 0.0f;
+}

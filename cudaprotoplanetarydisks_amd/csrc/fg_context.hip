@@ -219,6 +219,7 @@ extern "C" fg_status fg_destroy(fg_ctx *c) {
   if (!c) return FG_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fg_sg_release(c);
   void *ptrs[] = {c->t.Radii,      c->t.Rinf,       c->t.Rsup,        c->t.Rmed,         c->t.Surf,
                   c->t.invRinf,    c->t.invSurf,    c->t.invRmed,     c->t.invdiffRsup,  c->t.invdiffRmed,
                   c->t.SigmaMed,   c->t.SigmaInf,   c->t.EnergyMed,   c->t.CoolingTimeMed, c->t.QplusMed,

@@ -632,6 +632,462 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   if (tid == 0) a.newdt[i] = red[0];
 }
 
+// ================================================== FP32 bulk mode ====
+// Same passes as above with the arithmetic in single precision.  The state
+// (Sigma, vrad, vtheta) and the transported quantities between passes stay
+// FP64; every FP32 result enters them as an increment added in FP64, so an
+// update smaller than one FP32 ulp of the field is not lost and the fields
+// do not random-walk by an ulp per substep.  Quantities whose FP32 rounding
+// would cancel catastrophically are kept in FP64:
+//   - star gravity against the centrifugal term in the radial momentum
+//     equation (both ~1/r^2, their sum ~h^2/r^2): formed in FP64 per cell
+//     from per-ring values and the cell's vtheta offset;
+//   - vtheta is staged in LDS as the offset from a per-ring reference
+//     (the ring mean of the previous substep), a few 1e-3 of |vtheta|;
+//   - cell-planet separations are differenced in FP64;
+//   - the FARGO ring mean, Ntilde/Nshift and the uniform residual, the CFL
+//     count and the disk force are the FP64 code of the exact path.
+#define TH32 16
+#define HR32 (TH32 + HALO_R + HALO_B)
+#define SRC32_PER_THREAD ((HR32 * HC + SRC_TPB - 1) / SRC_TPB)
+
+struct src32_args {
+  const double *dens, *vr, *vt;
+  const double *vref;  // per-ring reference vtheta
+  double *odens, *ovr, *ovt, *opress;
+  const double *nu;
+  fg_dev_tables t;
+  fg_planets_arg P;
+  double dt, omega_frame, domega;
+  int nr, ns, indirect, evanescent, corotating;
+};
+
+// planet part of the potential at one cell (the star's -1/r is applied per
+// ring by the caller); Kernels.cu:1234-1252 with the separation in FP64
+__device__ __forceinline__ float cell_potential_planets_f(const fg_planets_arg &P, double Rm, double cosp,
+                                                          double sinp, int indirect) {
+  const double x = Rm * cosp, y = Rm * sinp;
+  float acc = 0.0f;
+  for (int k = 0; k < P.nb; k++) {
+    const float dx = (float)(x - P.x[k]), dy = (float)(y - P.y[k]);
+    const float m = (float)P.m[k];
+    float pv = -m * __builtin_amdgcn_rsqf(dx * dx + dy * dy + (float)P.smooth[k]);
+    if (indirect) pv += m * (float)P.invd3[k] * (float)(x * P.x[k] + y * P.y[k]);
+    acc += pv;
+    if (k == 0) acc -= (float)(P.ix * x + P.iy * y);
+  }
+  return acc;
+}
+
+// state of one cell after CorrectVtheta and the first damping pass (FP64)
+__device__ __forceinline__ void src32_cell_state(const src32_args &a, int i, long l, double &s, double &vr,
+                                                 double &vt) {
+  const fg_dev_tables &t = a.t;
+  s = a.dens[l];
+  vr = a.vr[l];
+  vt = a.vt[l];
+  if (i < a.nr) {
+    if (a.corotating) vt = vt - a.domega * t.Rmed[i];
+    if (a.evanescent && t.ev_zone[i]) {
+      double lambda, vrad0, vtheta0;
+      ev_consts(t, a.nu, i, a.dt, a.omega_frame, lambda, vrad0, vtheta0);
+      vr = damp_toward(vr, lambda, vrad0);
+      vt = damp_toward(vt, lambda, vtheta0);
+      s = damp_toward(s, lambda, t.SigmaMed[i]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
+  __shared__ float sS[HR32][HC];  // Sigma after the first damping
+  __shared__ float sA[HR32][HC];  // pressure, then VradInt
+  __shared__ float sB[HR32][HC];  // planet potential, then VthetaInt - vref
+  __shared__ float sC[HR32][HC];  // vtheta - vref, then TAURR, then Qr
+  __shared__ float sD[HR32][HC];  // vrad, then TAUPP, then Qtheta
+  __shared__ float sE[HR32][HC];  // TAURP
+  __shared__ float incR[TH32][TW], incT[TH32][TW];  // own-cell velocity increments
+  const int nr = a.nr, ns = a.ns;
+  const int i0 = blockIdx.y * TH32, j0 = blockIdx.x * TW;
+  const int tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const double dt = a.dt;
+  const float dtf = (float)dt;
+#define GI(r) (i0 - HALO_R + (r))
+#define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
+#define VREF(i) (((i) >= 0 && (i) < nr) ? a.vref[i] : 0.0)
+
+  // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
+  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
+    if (i >= 0 && i <= nr) {
+      int j = GJ(cc);
+      double sd, vrd, vtd;
+      src32_cell_state(a, i, (long)i * ns + j, sd, vrd, vtd);
+      s = (float)sd;
+      vr = (float)vrd;
+      vt = (float)(vtd - VREF(i));
+      if (i < nr) {
+        const double cs = t.cs_iso[i];
+        p = (float)(sd * cs * cs);
+        phi = cell_potential_planets_f(a.P, t.Rmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
+      }
+    }
+    sS[r][cc] = s;
+    sA[r][cc] = p;
+    sB[r][cc] = phi;
+    sC[r][cc] = vt;
+    sD[r][cc] = vr;
+  }
+  __syncthreads();
+
+  // ---- S1: Substep1 (Kernels.cu:13-53) ----
+  const bool star = a.P.nb > 0;  // the star's -1/r enters with the first planet
+  float regA[SRC32_PER_THREAD], regB[SRC32_PER_THREAD];
+  {
+    int k = 0;
+    for (int q = tid; q < HR32 * HC; q += SRC_TPB, k++) {
+      int r = q / HC, cc = q - r * HC;
+      int i = GI(r);
+      float vri = 0.f, vti = 0.f, dvr = 0.f, dvt = 0.f;
+      if (r >= 1 && cc >= 1 && cc < HC - 1 && i >= 0 && i < nr) {
+        if (i > 0) {
+          const float idr = (float)t.invdiffRmed[i];
+          float gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0f * rcp_f(sS[r][cc] + sS[r - 1][cc]) * idr;
+          float gradphi = (sB[r][cc] - sB[r - 1][cc]) * idr;
+          // centrifugal minus star gravity in FP64: vbar = ring part + offset
+          double vbar = 0.5 * (VREF(i) + VREF(i - 1)) + a.omega_frame * t.Rinf[i];
+          vbar += (double)((sC[r][cc] + sC[r][cc + 1] + sC[r - 1][cc] + sC[r - 1][cc + 1]) * 0.25f);
+          double bal = vbar * vbar * t.invRinf[i];
+          if (star) bal -= (t.invRmed[i - 1] - t.invRmed[i]) * t.invdiffRmed[i];
+          dvr = dtf * (-gradp - gradphi + (float)bal);
+          vri = sD[r][cc] + dvr;
+        }
+        const float invdxtheta = (float)t.invdxtheta[i];
+        float gradp = (sA[r][cc] - sA[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * invdxtheta;
+        float gradphi = (sB[r][cc] - sB[r][cc - 1]) * invdxtheta;
+        dvt = dtf * ((float)t.supp_torque[i] - (gradp + gradphi));
+        vti = sC[r][cc] + dvt;
+      }
+      regA[k] = vri;
+      regB[k] = vti;
+      const int ro = r - HALO_R, co = cc - HALO_C;
+      if (ro >= 0 && ro < TH32 && co >= 0 && co < TW) {
+        incR[ro][co] = dvr;
+        incT[ro][co] = dvt;
+      }
+    }
+    __syncthreads();
+    k = 0;
+    for (int q = tid; q < HR32 * HC; q += SRC_TPB, k++) {
+      int r = q / HC, cc = q - r * HC;
+      sA[r][cc] = regA[k];
+      sB[r][cc] = regB[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- S2: viscous stress (Kernels.cu:733-771) ----
+  const float invdphi = (float)t.invdphi;
+  const float onethird = 1.0f / 3.0f;
+  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 1 || r >= HR32 - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
+    const float iRm = (float)t.invRmed[i], idRs = (float)t.invdiffRsup[i];
+    const float vr0 = sA[r][cc], vr1 = sA[r + 1][cc];
+    const float dvt = sB[r][cc + 1] - sB[r][cc];
+    float drr = (vr1 - vr0) * idRs;
+    float dpp = dvt * invdphi * iRm + 0.5f * (vr1 + vr0) * iRm;
+    float dv = (vr1 * (float)t.Rsup[i] - vr0 * (float)t.Rinf[i]) * idRs * iRm;
+    dv += dvt * invdphi * iRm;
+    const float nu2s = 2.0f * sS[r][cc] * (float)a.nu[i];
+    sC[r][cc] = nu2s * (drr - onethird * dv);
+    sD[r][cc] = nu2s * (dpp - onethird * dv);
+    if (i > 0) {
+      // d(vtheta/r)/dr with the reference part of vtheta differenced in FP64
+      const double dref = (VREF(i) * t.invRmed[i] - VREF(i - 1) * t.invRmed[i - 1]);
+      const float dom = (float)dref + (sB[r][cc] * iRm - sB[r - 1][cc] * (float)t.invRmed[i - 1]);
+      float drp = 0.5f * ((float)t.Rinf[i] * dom * (float)t.invdiffRmed[i] +
+                          (sA[r][cc] - sA[r][cc - 1]) * invdphi * (float)t.invRinf[i]);
+      sE[r][cc] = 0.5f * (sS[r][cc] + sS[r - 1][cc] + sS[r][cc - 1] + sS[r - 1][cc - 1]) * (float)a.nu[i] * drp;
+    }
+  }
+  __syncthreads();
+
+  // ---- S3: viscous velocity update (Kernels.cu:118-150) ----
+  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 2 || r >= HR32 - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
+    const float iRm = (float)t.invRmed[i];
+    const int ro = r - HALO_R, co = cc - HALO_C;
+    const bool own = ro >= 0 && ro < TH32 && co >= 0 && co < TW;
+    if (i < nr - 1 && r < HR32 - 2) {
+      float d = dtf * iRm *
+                (((float)t.Rsup[i] * sE[r + 1][cc] - (float)t.Rinf[i] * sE[r][cc]) * (float)t.invdiffRsup[i] +
+                 (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5f * (sE[r][cc] + sE[r + 1][cc])) *
+                rcp_f(0.5f * (sS[r][cc] + sS[r][cc - 1]));
+      sB[r][cc] += d;
+      if (own) incT[ro][co] += d;
+    }
+    float d = dtf * (float)t.invRinf[i] *
+              (((float)t.Rmed[i] * sC[r][cc] - (float)t.Rmed[i - 1] * sC[r - 1][cc]) * (float)t.invdiffRmed[i] +
+               (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5f * (sD[r][cc] + sD[r - 1][cc])) *
+              rcp_f(0.5f * (sS[r][cc] + sS[r - 1][cc]));
+    sA[r][cc] += d;
+    if (own) incR[ro][co] += d;
+  }
+  __syncthreads();
+
+  // ---- S4: artificial pressure (Kernels.cu:409-434) ----
+  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+    int r = q / HC, cc = q - r * HC;
+    int i = GI(r);
+    if (r < 3 || r >= HR32 - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
+    sC[r][cc] = art_visc_q_f(sS[r][cc], sA[r + 1][cc] - sA[r][cc]);
+    sD[r][cc] = art_visc_q_f(sS[r][cc], sB[r][cc + 1] - sB[r][cc]);
+  }
+  __syncthreads();
+
+  // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
+  for (int q = tid; q < TH32 * TW; q += SRC_TPB) {
+    int ro = q / TW, co = q % TW;
+    int r = ro + HALO_R, cc = co + HALO_C;
+    int i = GI(r), j = j0 + co;
+    if (i > nr || j >= ns) continue;
+    long l = (long)i * ns + j;
+    double s, vr, vt;
+    src32_cell_state(a, i, l, s, vr, vt);
+    if (a.opress && i < nr) a.opress[l] = s * t.cs_iso[i] * t.cs_iso[i];
+    if (i == nr) {  // ghost ring
+      a.odens[l] = s;
+      a.ovr[l] = 0.0;
+      a.ovt[l] = 0.0;
+      continue;
+    }
+    double vrnew = 0.0;
+    if (i > 0) {
+      float dq = (sC[r][cc] - sC[r - 1][cc]) * 2.0f * rcp_f(sS[r][cc] + sS[r - 1][cc]) * (float)t.invdiffRmed[i];
+      vrnew = vr + (double)(incR[ro][co] - dtf * dq);
+    }
+    float dq = (sD[r][cc] - sD[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * (float)t.invdxtheta[i];
+    double vtnew = vt + (double)(incT[ro][co] - dtf * dq);
+    if (a.evanescent && t.ev_zone[i]) {
+      double lambda, vrad0, vtheta0;
+      ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
+      vrnew = damp_toward(vrnew, lambda, vrad0);
+      vtnew = damp_toward(vtnew, lambda, vtheta0);
+      s = damp_toward(s, lambda, t.SigmaMed[i]);
+    }
+    a.odens[l] = s;
+    a.ovr[l] = vrnew;
+    a.ovt[l] = vtnew;
+  }
+#undef GI
+#undef GJ
+#undef VREF
+}
+
+// Radial sweep in FP32 (k_trad above): Q of each ring is formed in FP64,
+// its Work value, slopes, face values and fluxes in FP32; the flux
+// divergence is added to the FP64 Q.
+__global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) {
+  const int nr = a.nr, ns = a.ns;
+  if ((int)blockIdx.x < nr) {
+    ring_mean_wave(a.vt + (long)blockIdx.x * ns, ns, a.vmed + blockIdx.x);
+    return;
+  }
+  const int bid = blockIdx.x - nr;
+  const int j = (bid % a.nstrips) * TRAD_TPB + threadIdx.x;
+  if (j >= ns) return;
+  const int jp = (j + 1) % ns;
+  const int b0 = (bid / a.nstrips) * TRAD_BAND;
+  const int b1 = min(b0 + TRAD_BAND, nr);
+  const fg_dev_tables &t = a.t;
+  const double dt = a.dt, OF = a.omega_frame;
+  const float dtf = (float)dt;
+  const float cflux = (float)(dt * t.dtheta);
+  double Q[NQ][3] = {};
+  float W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
+  float sg[3] = {}, dqs[2] = {}, vr[3] = {};
+  int m0 = max(b0 - 2, 0);
+  for (int m = m0; m <= b1 + 1; m++) {
+    for (int q = 0; q < NQ; q++) {
+      Q[q][0] = Q[q][1];
+      Q[q][1] = Q[q][2];
+      W[q][0] = W[q][1];
+      W[q][1] = W[q][2];
+    }
+    sg[0] = sg[1];
+    sg[1] = sg[2];
+    vr[0] = vr[1];
+    vr[1] = vr[2];
+    if (m < nr) {
+      const long l = (long)m * ns + j;
+      const double s = a.dens[l];
+      const double v0 = a.vr[l], v1 = a.vr[l + ns];
+      const double Rm = t.Rmed[m];
+      Q[0][2] = s * v1;
+      Q[1][2] = s * v0;
+      Q[2][2] = s * (a.vt[(long)m * ns + jp] + Rm * OF) * Rm;
+      Q[3][2] = s * (a.vt[l] + Rm * OF) * Rm;
+      Q[4][2] = s;
+      const float fs = (float)s;
+      const float rs = rcp_f(fs + 1e-20f);
+      for (int q = 0; q < NQ; q++) W[q][2] = (float)Q[q][2] * rs;
+      sg[2] = fs;
+      vr[2] = (float)v0;
+    } else {
+      for (int q = 0; q < NQ; q++) {
+        Q[q][2] = 0.0;
+        W[q][2] = 0.0f;
+      }
+      sg[2] = 0.0f;
+      vr[2] = 0.0f;
+    }
+    const int k = m - 1;
+    if (k >= 0) {
+      const bool flat = (k == 0 || k >= nr - 1);
+      const float idm0 = flat ? 0.f : (float)t.invdiffRmed[k], idm1 = flat ? 0.f : (float)t.invdiffRmed[k + 1];
+      for (int q = 0; q < NQ; q++) {
+        dq[q][0] = dq[q][1];
+        dq[q][1] = flat ? 0.0f : vl_rad_slope_f(W[q][0], W[q][1], W[q][2], idm0, idm1);
+      }
+      dqs[0] = dqs[1];
+      dqs[1] = flat ? 0.0f : vl_rad_slope_f(sg[0], sg[1], sg[2], idm0, idm1);
+      float DS = 0.0f, QS[NQ];
+      if (k > 0 && k < nr) {
+        const float vk = vr[1];
+        const float dm = (float)(t.Rmed[k] - t.Rmed[k - 1]), dp = (float)(t.Rmed[k + 1] - t.Rmed[k]);
+        DS = vl_rad_star_f(vk, dtf, sg[0], sg[1], dqs[0], dqs[1], dm, dp);
+        for (int q = 0; q < NQ; q++) QS[q] = vl_rad_star_f(vk, dtf, W[q][0], W[q][1], dq[q][0], dq[q][1], dm, dp);
+      } else {
+        for (int q = 0; q < NQ; q++) QS[q] = 0.0f;
+      }
+      const float vface = (k < nr) ? vr[1] : 0.0f;
+      const float cf = cflux * (float)t.Rinf[k] * DS * vface;
+      for (int q = 0; q < NQ; q++) {
+        F[q][0] = F[q][1];
+        F[q][1] = cf * QS[q];
+      }
+      const int o = k - 1;
+      if (o >= b0 && o < b1) {
+        const long lo = (long)o * ns + j;
+        const float isurf = (float)t.invSurf[o];
+        for (int q = 0; q < NQ; q++) a.out[q][lo] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
+      }
+    }
+  }
+}
+
+// Azimuthal sweep in FP32 (azimuthal_sweep above); q stays FP64.
+template <int C, bool UNIFORM>
+__device__ __forceinline__ void azimuthal_sweep32(double (&q)[NQ][C], const float (&Vc)[C], float Vu, float *W,
+                                                  float *D, int ns, int i, const fg_dev_tables &t, float dt) {
+  const float invdxtheta = (float)t.invdxtheta[i], dxtheta = (float)t.dxtheta[i];
+  const float dxrad = (float)((t.Rsup[i] - t.Rinf[i]) * (double)dt);
+  const float invsurf = (float)t.invSurf[i];
+  float rs[C], DS[C];
+#define V(k) (UNIFORM ? Vu : Vc[k])
+#define FOR_CELLS(k, j) \
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+  FOR_CELLS(k, j) {
+    const float d = (float)q[NQ - 1][k];
+    rs[k] = rcp_f(d + 1e-20f);
+    W[j] = d;
+  }
+  __syncthreads();
+  FOR_CELLS(k, j) D[j] = vl_theta_slope_f(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
+  __syncthreads();
+  FOR_CELLS(k, j) {
+    int jm = j == 0 ? ns - 1 : j - 1;
+    DS[k] = vl_theta_star_f(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta) * V(k) * dxrad;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int qq = 0; qq < NQ; qq++) {
+    FOR_CELLS(k, j) W[j] = (float)q[qq][k] * rs[k];
+    __syncthreads();
+    FOR_CELLS(k, j) D[j] = vl_theta_slope_f(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
+    __syncthreads();
+    float F[C];
+    FOR_CELLS(k, j) {
+      int jm = j == 0 ? ns - 1 : j - 1;
+      F[k] = vl_theta_star_f(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta) * DS[k];
+    }
+    __syncthreads();
+    FOR_CELLS(k, j) W[j] = F[k];
+    __syncthreads();
+    FOR_CELLS(k, j) q[qq][k] += (double)((F[k] - W[j + 1 == ns ? 0 : j + 1]) * invsurf);
+    __syncthreads();
+  }
+#undef V
+#undef FOR_CELLS
+}
+
+template <int C>
+__global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
+  extern __shared__ double lds[];
+  float *Wf = (float *)lds, *Df = Wf + a.ns;
+  double *W = lds, *D = lds + a.ns;
+  const int ns = a.ns, i = blockIdx.x, tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const long base = (long)i * ns;
+  const double VMed = a.vmed[i];
+  double q[NQ][C];
+  float vres[C];
+#define FOR_CELLS(k, j) \
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+  // FARGO residual phase in FP64 (Kernels.cu:1084-1126)
+  const double invdt = 1.0 / a.dt;
+  const double dpinvns = 2.0 * FG_PI / (double)ns;
+  const double Ntilde = VMed * t.invRmed[i] * a.dt * (double)ns / 2.0 / FG_PI;
+  const double Nround = floor(Ntilde + 0.5);
+  const int nsh = (int)(long)Nround;
+  double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
+  FOR_CELLS(k, j) {
+    double vr = a.vt[base + j] - VMed;
+    if (!a.fast) vr += vuni;
+    vres[k] = (float)vr;
+#pragma unroll
+    for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
+  }
+  if (!a.fast) vuni = 0.0;
+  if (tid == 0) a.nshift[i] = nsh;
+  azimuthal_sweep32<C, false>(q, vres, 0.0f, Wf, Df, ns, i, t, (float)a.dt);
+  azimuthal_sweep32<C, true>(q, vres, (float)vuni, Wf, Df, ns, i, t, (float)a.dt);
+  int sh = nsh % ns;
+  if (sh < 0) sh += ns;
+#pragma unroll
+  for (int qq = 0; qq < NQ; qq++) {
+    FOR_CELLS(k, j) W[j] = q[qq][k];
+    __syncthreads();
+    FOR_CELLS(k, j) {
+      int ji = j - sh;
+      q[qq][k] = W[ji < 0 ? ji + ns : ji];
+    }
+    __syncthreads();
+  }
+  FOR_CELLS(k, j) {
+    W[j] = q[2][k];
+    D[j] = q[NQ - 1][k];
+  }
+  __syncthreads();
+  const double Rm = t.Rmed[i];
+  FOR_CELLS(k, j) {
+    int jm = j == 0 ? ns - 1 : j - 1;
+    double s = q[NQ - 1][k];
+    double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
+    a.xd[base + j] = s;
+    a.xvt[base + j] = v;
+    a.xrmp[base + j] = q[0][k];
+    a.xrmm[base + j] = q[1][k];
+  }
+#undef FOR_CELLS
+}
+
 // ----------------------------------------------------- entry points ----
 #define PROF_KERNELS 6
 
@@ -821,8 +1277,34 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     c->prof_nev = 0;
     c->prof_pending = 1;
   }
+  const bool fp32 = c->p.precision == FG_PREC_FP32;
   prof_mark(c);
-  hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
+  if (fp32) {
+    src32_args s3;
+    s3.dens = sa.dens;
+    s3.vr = sa.vr;
+    s3.vt = sa.vt;
+    s3.vref = c->Vmoy;  // ring means of vtheta from the previous substep
+    s3.odens = sa.odens;
+    s3.ovr = sa.ovr;
+    s3.ovt = sa.ovt;
+    s3.opress = sa.opress;
+    s3.nu = sa.nu;
+    s3.t = sa.t;
+    s3.P = sa.P;
+    s3.dt = dt;
+    s3.omega_frame = omega_frame;
+    s3.domega = domega;
+    s3.nr = nr;
+    s3.ns = ns;
+    s3.indirect = sa.indirect;
+    s3.evanescent = sa.evanescent;
+    s3.corotating = sa.corotating;
+    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + TH32 - 1) / TH32), dim3(SRC_TPB), 0, c->stream,
+                       s3);
+  } else {
+    hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
+  }
   FG_LAUNCH_CHECK();
   // k_trad
   trad_args ta;
@@ -842,8 +1324,11 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.vmed = c->VMed;
   ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
   prof_mark(c);
-  hipLaunchKernelGGL(k_trad, dim3(nr + ta.nstrips * ((nr + TRAD_BAND - 1) / TRAD_BAND)), dim3(TRAD_TPB), 0,
-                     c->stream, ta);
+  const dim3 gtrad(nr + ta.nstrips * ((nr + TRAD_BAND - 1) / TRAD_BAND));
+  if (fp32)
+    hipLaunchKernelGGL(k_trad32, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
+  else
+    hipLaunchKernelGGL(k_trad, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
   FG_LAUNCH_CHECK();
   // k_taz
   taz_args za;
@@ -868,16 +1353,23 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   prof_mark(c);
   const size_t shm = 2 * (size_t)ns * sizeof(double);
   const dim3 g(nr), blk(TAZ_TPB);
-  switch (taz_cells(ns)) {
-    case 1: hipLaunchKernelGGL(k_taz<1>, g, blk, shm, c->stream, za); break;
-    case 2: hipLaunchKernelGGL(k_taz<2>, g, blk, shm, c->stream, za); break;
-    case 3: hipLaunchKernelGGL(k_taz<3>, g, blk, shm, c->stream, za); break;
-    case 4: hipLaunchKernelGGL(k_taz<4>, g, blk, shm, c->stream, za); break;
-    case 6: hipLaunchKernelGGL(k_taz<6>, g, blk, shm, c->stream, za); break;
-    case 8: hipLaunchKernelGGL(k_taz<8>, g, blk, shm, c->stream, za); break;
-    case 12: hipLaunchKernelGGL(k_taz<12>, g, blk, shm, c->stream, za); break;
-    default: hipLaunchKernelGGL(k_taz<16>, g, blk, shm, c->stream, za); break;
+#define TAZ_LAUNCH(K)                                                       \
+  switch (taz_cells(ns)) {                                                  \
+    case 1: hipLaunchKernelGGL(K<1>, g, blk, shm, c->stream, za); break;   \
+    case 2: hipLaunchKernelGGL(K<2>, g, blk, shm, c->stream, za); break;   \
+    case 3: hipLaunchKernelGGL(K<3>, g, blk, shm, c->stream, za); break;   \
+    case 4: hipLaunchKernelGGL(K<4>, g, blk, shm, c->stream, za); break;   \
+    case 6: hipLaunchKernelGGL(K<6>, g, blk, shm, c->stream, za); break;   \
+    case 8: hipLaunchKernelGGL(K<8>, g, blk, shm, c->stream, za); break;   \
+    case 12: hipLaunchKernelGGL(K<12>, g, blk, shm, c->stream, za); break; \
+    default: hipLaunchKernelGGL(K<16>, g, blk, shm, c->stream, za); break; \
   }
+  if (fp32) {
+    TAZ_LAUNCH(k_taz32)
+  } else {
+    TAZ_LAUNCH(k_taz)
+  }
+#undef TAZ_LAUNCH
   FG_LAUNCH_CHECK();
   // k_fin
   fin_args fa;

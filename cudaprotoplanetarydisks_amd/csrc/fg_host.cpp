@@ -740,6 +740,49 @@ fg_status begin_dt(fg_sim *s) {
 
 }  // namespace
 
+// Self-gravity of the initial disk (Main.cu:268-290): the field of the
+// uploaded density, the planets' orbital speed corrected by the ring mean
+// of the radial SG acceleration (Init_planetarysys_withSG,
+// SgMain.cu:166-194), the frame rate recomputed from them, and the gas
+// rotation in equilibrium with it (Init_azimutalvelocity_withSG,
+// SourceEuler.cu:441-450, Kernels.cu:935-950) minus the frame rotation
+// (Kernels.cu:648-689).
+static fg_status sg_initial_state(fg_sim *s, std::vector<double> &vtheta) {
+  const Deck &d = s->d;
+  const int nr = s->nr, ns = s->ns;
+  CHECK(fg_selfgravity(s->ctx, 0.0, 0));
+  std::vector<double> axi(nr + 1, 0.0);
+  CHECK(fg_get_rings(s->ctx, nullptr, nullptr, nullptr, nullptr, axi.data()));
+  Planets &p = s->ps;
+  for (int k = 0; k < p.nb; k++) {
+    double r = p.x[k];
+    double dist = (double)(r / (1. + d.ECCENTRICITY));
+    int ipl = 0;
+    while (ipl < nr - 2 && s->Rmed[ipl] <= dist) ipl++;
+    double ri = s->Rmed[ipl], rip1 = s->Rmed[ipl + 1], dr = rip1 - ri;
+    double sgacc = (dist - ri) * axi[ipl + 1] + (rip1 - dist) * axi[ipl];
+    sgacc /= dr;
+    p.vy[k] *= (double)sqrt(1. - dist * dist * sgacc);
+  }
+  s->OmegaFrame = d.OMEGAFRAME;
+  if (d.Corotating) s->OmegaFrame = psys_info(s, 2);
+  for (int i = 0; i <= nr; i++) {
+    double v = 0.0, r = (i == nr) ? s->Rmed[nr - 1] : s->Rmed[i];
+    if (i < nr) {
+      double invr = 1. / s->Rmed[i];
+      double omegakep = sqrt(G * 1.0 * invr * invr * invr);
+      double omega = sqrt(omegakep * omegakep *
+                              (1.0 - (1. + d.SIGMASLOPE - 2.0 * d.FLARINGINDEX) * pow(d.ASPECTRATIO, 2.0) *
+                                         pow(s->Rmed[i], 2.0 * d.FLARINGINDEX)) -
+                          invr * axi[i]);
+      v = s->Rmed[i] * omega;
+    }
+    v -= s->OmegaFrame * r;
+    for (int j = 0; j < ns; j++) vtheta[(size_t)i * ns + j] = v;
+  }
+  return fg_set_field(s->ctx, FG_VTHETA, vtheta.data());
+}
+
 // -------------------------------------------------------------- C-ABI --
 extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *opt, fg_sim **out) {
   if (!parfile || !out) {
@@ -806,11 +849,6 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
     for (int i = 0; i < nr; i++) s->EnergyMed[i] = s->prof.energy(s->Rmed[i]);
     for (int i = 0; i < nr; i++)
       for (int j = 0; j < ns; j++) energy[(size_t)i * ns + j] = s->EnergyMed[i];
-  }
-  if (d.SelfGravity) {
-    host_error("self-gravity decks need the FFT path, which this build does not provide");
-    delete s;
-    return FG_E_UNSUPPORTED;
   }
   s->OmegaFrame = d.OMEGAFRAME;
   if (d.Corotating) s->OmegaFrame = psys_info(s, 2);
@@ -880,11 +918,19 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
   p.alphaviscosity = d.ALPHAVISCOSITY;
   p.sigmaslope = d.SIGMASLOPE;
   p.flaringindex = d.FLARINGINDEX;
+  // self-gravity constants, Main.cu:269-271
+  p.sg_eps = d.THICKNESSSMOOTHING * d.ASPECTRATIO;
+  p.sg_rstep = log(s->Radii[nr] / s->Radii[0]) / (double)nr;
+  p.sg_tstep = 2.0 * PI / (double)ns;
   if ((st = fg_create(&p, &tables, s->opt.device, &s->ctx))) {
     delete s;
     return st;
   }
   if ((st = fg_upload_state(s->ctx, dens.data(), vrad.data(), vtheta.data(), d.Adiabatic ? energy.data() : nullptr))) {
+    fg_sim_destroy(s);
+    return st;
+  }
+  if (d.SelfGravity && (st = sg_initial_state(s, vtheta))) {
     fg_sim_destroy(s);
     return st;
   }
@@ -914,8 +960,16 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
   s->prof.time0 = s->PhysicalTimeInitial;
   // the fused path covers locally isothermal decks with evanescent edges
   // and a per-substep CFL; anything else stays on the per-routine path
-  s->fused_active = s->opt.fused && !d.Adiabatic && !d.SelfGravity && d.Evanescent && !d.OpenInner &&
-                    !d.NonReflecting && !d.SloppyCFL && d.IsDisk && ns <= 4096;
+  const bool fusable = !d.Adiabatic && !d.SelfGravity && d.Evanescent && !d.OpenInner && !d.NonReflecting &&
+                       !d.SloppyCFL && d.IsDisk && ns <= 4096;
+  // the FP32 bulk mode exists only as fused kernels
+  if (s->opt.precision == FG_PREC_FP32 && !fusable) {
+    host_error("FP32 bulk mode needs a locally isothermal deck with evanescent edges, per-substep CFL and "
+               "NSEC <= 4096");
+    fg_sim_destroy(s);
+    return FG_E_UNSUPPORTED;
+  }
+  s->fused_active = (s->opt.fused || s->opt.precision == FG_PREC_FP32) && fusable;
   *out = s;
   return FG_OK;
 }

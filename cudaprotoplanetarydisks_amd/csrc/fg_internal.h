@@ -50,6 +50,11 @@ struct fg_dev_tables {
   double dtheta, invdphi, dt1d_coef, vrad0_slope, visc_value;
 };
 
+// mixed-radix FFT plan of one transform length (fg_sg.hip)
+#define FG_FFT_MAXST 40
+struct fg_fft_plan;
+void fg_sg_release(struct fg_ctx *c);
+
 struct fg_planets_arg {
   int nb;
   double x[FG_MAX_PLANETS], y[FG_MAX_PLANETS], m[FG_MAX_PLANETS], smooth[FG_MAX_PLANETS],
@@ -82,6 +87,7 @@ struct fg_ctx {
   // self-gravity complex buffers: 2*nr x ns interleaved complex
   double *sg_Sr, *sg_St, *sg_Krc, *sg_Ktc, *sg_Gr, *sg_Gt, *sg_tmp;
   int sg_kernel_ready;
+  fg_fft_plan *fft_row, *fft_col;  // FFT plans of the NSEC rows and 2*NRAD columns
   // FP32 bulk mirrors (precision mode)
   float *f32[8];
   // host copies of ring values the host-side parts of a routine need

@@ -16,39 +16,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long-running case")
 
 
-BASE_DECK = {
-    "AspectRatio": "0.05", "Sigma0": "6.3662e-4", "Viscosity": "1e-5", "SigmaSlope": "0.0",
-    "FlaringIndex": "0.0", "PlanetConfig": os.path.join(DECKS, "jupiter_fixed.cfg"),
-    "ThicknessSmoothing": "0.6", "Transport": "FARGO", "InnerBoundary": "EVANESCENT", "Disk": "YES",
-    "OmegaFrame": "1.0", "Frame": "GUIDING-CENTER", "IndirectTerm": "YES", "Nrad": "128", "Nsec": "384",
-    "Rmin": "0.4", "Rmax": "2.5", "RadialSpacing": "LOGARITHM", "Ntot": "199", "Ninterm": "20",
-    "DT": "0.314159265359", "OutputDir": "out/", "SelfGravity": "NO", "Adiabatic": "NO", "Cooling": "NO",
-}
-
-
-def write_deck(path, **overrides):
-    """Write a .par deck: the plain-FARGO workload with overrides
-    (value None drops the key)."""
-    d = dict(BASE_DECK)
-    for k, v in overrides.items():
-        if v is None:
-            d.pop(k, None)
-        else:
-            d[k] = str(v)
-    with open(path, "w") as f:
-        f.write("# deck written by the test suite\n")
-        for k, v in d.items():
-            f.write(f"{k}\t\t{v}\n")
-    return str(path)
-
-
-def write_planets(path, rows):
-    """rows: (name, dist, mass, accretion, feeldisk, feelothers)"""
-    with open(path, "w") as f:
-        f.write("# name dist mass accretion feeldisk feelothers\n")
-        for r in rows:
-            f.write("\t".join(str(x) for x in r) + "\n")
-    return str(path)
+from cudaprotoplanetarydisks_amd.workloads import BASE_DECK, write_deck, write_planets  # noqa: E402,F401
 
 
 @pytest.fixture
