@@ -1,7 +1,8 @@
 """Build libfargo_hip.so and the fargo_mi355x executable in-tree.
 
 hipcc compiles the HIP sources for gfx950 only; g++ compiles the host
-driver.  Both use -ffp-contract=off so that no multiply-add is fused:
+driver and the host tables.  Both use -ffp-contract=off so that no
+multiply-add is fused:
 the FP64 path rounds every operation like the reference's expression
 order.  Run:  python -m cudaprotoplanetarydisks_amd.build
 """
@@ -24,10 +25,13 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
+# cos and sin stay separate libm calls (no fused sincos, which can differ
+# from sin in the last bit): the oracle is built the same way
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos", "-Wall",
+             f"-I{INCLUDE}"]
 
 HIP_SOURCES = ["fg_context.hip", "fg_steps.hip", "fg_reduce.hip", "fg_sg.hip", "fg_fused.hip"]
-HOST_SOURCES = ["fg_host.cpp"]
+HOST_SOURCES = ["fg_host.cpp", "fg_hostmath.cpp"]
 
 
 def _run(cmd):

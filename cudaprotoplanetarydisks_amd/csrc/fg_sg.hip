@@ -21,6 +21,8 @@
 // transformed once (the reference redoes this every call).
 #include "fg_internal.h"
 
+extern "C" void fg_fft_twiddles(int L, int sign, double *cs);  // fg_hostmath.cpp
+
 #define FFT_TPB 256
 #define FFT_U 16  // outputs per thread: N * lines <= FFT_TPB * FFT_U
 #define FFT_MAX_ELEMS (FFT_TPB * FFT_U)
@@ -197,11 +199,7 @@ static fg_status plan_create(int n, fg_fft_plan **out) {
   for (int sg = 0; sg < 2; sg++) {
     const int sign = sg == 0 ? -1 : 1;
     std::vector<double2> tw((size_t)(total > 0 ? total : 1));
-    for (int d = 0; d < k; d++)
-      for (int q = 0; q < pl->L[d]; q++) {
-        double ang = sign * 2.0 * FG_PI * (double)q / (double)pl->L[d];
-        tw[pl->off[d] + q] = make_double2(cos(ang), sin(ang));
-      }
+    for (int d = 0; d < k; d++) fg_fft_twiddles(pl->L[d], sign, (double *)&tw[pl->off[d]]);
     hipError_t e = hipMalloc((void **)&pl->tw[sg], tw.size() * sizeof(double2));
     if (e == hipSuccess) e = hipMemcpy(pl->tw[sg], tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
