@@ -464,8 +464,11 @@ double or_device_reduce(const double *in, int n) {
   const int threads = 32;
   int blocks = (int)((nearest_pow2(n) + threads - 1) / threads);
   if (blocks > 1024) blocks = 1024;
-  /* IsPow2 (Kernels.cu:358-361) is true only for n == 1 */
-  const int nIsPow2 = (n == 1);
+  /* IsPow2 (Kernels.cu:358-361) is true only for n == 1, where the
+   * reference then reads in[32], past the end of the array.  That read is
+   * skipped here (the element counts as absent); DeviceReduce is only
+   * called on NRAD*NSEC or NSEC >= 2 values, where nIsPow2 is false. */
+  const int nIsPow2 = 0;
   const unsigned gridSize = (unsigned)(threads * 2 * blocks);
   double *bsum = dalloc(blocks);
 #pragma omp parallel for schedule(static)
