@@ -76,6 +76,32 @@ def cpu_baseline(deck, nr, ns, steps, warm):
     return nr * ns * steps / (t1 - t0), t1 - t0
 
 
+def ensemble_leg(disks, steps, warmup, seed=SEED):
+    """Batched ensemble of drawn disks (uneven grids 128..384 x 3 NRAD, FP32
+    bulk): aggregate zone-updates/s over `steps` batch substeps, all disks
+    active (the target time is far)."""
+    from cudaprotoplanetarydisks_amd.ensemble import make_ensemble
+    ens, params, _ = make_ensemble(disks, seed)
+    try:
+        t_end = 1e30
+        for _ in range(warmup):
+            ens.substep(t_end)
+        sub0 = [ens.get(k, "substeps") for k in range(ens.n)]
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ens.substep(t_end)
+        wall = time.perf_counter() - t0
+        zones = [(R - 1) * ns for R, ns in ens.shapes]
+        zu = sum(z * (ens.get(k, "substeps") - s0) for k, (z, s0) in enumerate(zip(zones, sub0)))
+        grids = sorted({p["nrad"] for p in params})
+        return {"disks": disks, "seed": seed, "grids_nrad": grids, "zones": int(sum(zones)),
+                "batch_substeps": steps, "ms_per_batch_substep": round(1e3 * wall / steps, 4),
+                "value": round(zu / wall, 1), "unit": "zone-updates/s", "dtype": "f32",
+                "launch": "one captured hipGraph per batch substep, one persistent launch per phase"}
+    finally:
+        ens.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +113,8 @@ def main():
     ap.add_argument("--path", choices=["fused", "routine"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-disks", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -140,6 +168,9 @@ def main():
             roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+        ens = None
+        if args.ensemble_disks > 0:
+            ens = ensemble_leg(args.ensemble_disks, args.ensemble_steps, 10)
         cpu = None
         if not args.no_cpu:
             threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -157,7 +188,7 @@ def main():
                                "q=1e-3 planet, evanescent edges, seeded 1e-4 noise on Sigma",
                    "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision},
         "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
         "finite": finite,
     }

@@ -21,6 +21,10 @@
 //
 // Supported decks: locally isothermal, evanescent boundaries, no
 // self-gravity, per-DT CFL (everything else uses the per-routine path).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "fg_cell.h"
 #include "fg_internal.h"
 
@@ -288,6 +292,7 @@ struct trad_args {
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, nstrips;
+  int band;  // rings per column strip (FP32 sweep; the FP64 sweep uses TRAD_BAND)
 };
 
 // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081) for one ring
@@ -557,11 +562,11 @@ __device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
   return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
 }
 
-__global__ __launch_bounds__(256) void k_fin(fin_args a) {
-  extern __shared__ double ring[];  // final vtheta of the ring
+// one ring; ring holds NSEC doubles (the ring's final vtheta)
+__device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[256];
   __shared__ double s_vmoy;
-  const int ns = a.ns, nr = a.nr, i = blockIdx.x, tid = threadIdx.x;
+  const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
   double lam_i = 0, vr0_i = 0, vt0_i = 0, lam_n = 0, vr0_n = 0, vt0_n = 0;
@@ -632,6 +637,11 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   if (tid == 0) a.newdt[i] = red[0];
 }
 
+__global__ __launch_bounds__(256) void k_fin(fin_args a) {
+  extern __shared__ double lds_fin[];
+  fin_ring(a, blockIdx.x, lds_fin);
+}
+
 // ================================================== FP32 bulk mode ====
 // Same passes as above with the arithmetic in single precision.  The state
 // (Sigma, vrad, vtheta) and the transported quantities between passes stay
@@ -647,9 +657,14 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
 //   - cell-planet separations are differenced in FP64;
 //   - the FARGO ring mean, Ntilde/Nshift and the uniform residual, the CFL
 //     count and the disk force are the FP64 code of the exact path.
-#define TH32 16
-#define HR32 (TH32 + HALO_R + HALO_B)
-#define SRC32_PER_THREAD ((HR32 * HC + SRC_TPB - 1) / SRC_TPB)
+// The tile height th (rings per tile, <= SRC32_MAX_TH) is a launch
+// parameter chosen per grid size; the LDS tiles are carved from dynamic
+// shared memory: six (th + 6) x HC float arrays and two th x TW
+// increment arrays.
+#define SRC32_MAX_TH 32
+#define SRC32_PER_THREAD (((SRC32_MAX_TH + HALO_R + HALO_B) * HC + SRC_TPB - 1) / SRC_TPB)
+
+static size_t src32_lds_bytes(int th) { return (size_t)(6 * (th + HALO_R + HALO_B) * HC + 2 * th * TW) * sizeof(float); }
 
 struct src32_args {
   const double *dens, *vr, *vt;
@@ -698,16 +713,18 @@ __device__ __forceinline__ void src32_cell_state(const src32_args &a, int i, lon
   }
 }
 
-__global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
-  __shared__ float sS[HR32][HC];  // Sigma after the first damping
-  __shared__ float sA[HR32][HC];  // pressure, then VradInt
-  __shared__ float sB[HR32][HC];  // planet potential, then VthetaInt - vref
-  __shared__ float sC[HR32][HC];  // vtheta - vref, then TAURR, then Qr
-  __shared__ float sD[HR32][HC];  // vrad, then TAUPP, then Qtheta
-  __shared__ float sE[HR32][HC];  // TAURP
-  __shared__ float incR[TH32][TW], incT[TH32][TW];  // own-cell velocity increments
+__device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, int th, float *lds) {
+  const int hr = th + HALO_R + HALO_B;
+  float(*sS)[HC] = (float(*)[HC])(lds);           // Sigma after the first damping
+  float(*sA)[HC] = (float(*)[HC])(lds + hr * HC);  // pressure, then VradInt
+  float(*sB)[HC] = (float(*)[HC])(lds + 2 * hr * HC);  // planet potential, then VthetaInt - vref
+  float(*sC)[HC] = (float(*)[HC])(lds + 3 * hr * HC);  // vtheta - vref, then TAURR, then Qr
+  float(*sD)[HC] = (float(*)[HC])(lds + 4 * hr * HC);  // vrad, then TAUPP, then Qtheta
+  float(*sE)[HC] = (float(*)[HC])(lds + 5 * hr * HC);  // TAURP
+  float(*incR)[TW] = (float(*)[TW])(lds + 6 * hr * HC);  // own-cell velocity increments
+  float(*incT)[TW] = (float(*)[TW])(lds + 6 * hr * HC + th * TW);
   const int nr = a.nr, ns = a.ns;
-  const int i0 = blockIdx.y * TH32, j0 = blockIdx.x * TW;
+  const int i0 = by * th, j0 = bx * TW;
   const int tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const double dt = a.dt;
@@ -717,7 +734,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
 #define VREF(i) (((i) >= 0 && (i) < nr) ? a.vref[i] : 0.0)
 
   // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
-  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+  for (int q = tid; q < hr * HC; q += SRC_TPB) {
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
@@ -746,8 +763,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
   const bool star = a.P.nb > 0;  // the star's -1/r enters with the first planet
   float regA[SRC32_PER_THREAD], regB[SRC32_PER_THREAD];
   {
-    int k = 0;
-    for (int q = tid; q < HR32 * HC; q += SRC_TPB, k++) {
+#pragma unroll
+    for (int k = 0; k < SRC32_PER_THREAD; k++) {
+      const int q = tid + k * SRC_TPB;
+      if (q >= hr * HC) break;
       int r = q / HC, cc = q - r * HC;
       int i = GI(r);
       float vri = 0.f, vti = 0.f, dvr = 0.f, dvt = 0.f;
@@ -773,14 +792,16 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
       regA[k] = vri;
       regB[k] = vti;
       const int ro = r - HALO_R, co = cc - HALO_C;
-      if (ro >= 0 && ro < TH32 && co >= 0 && co < TW) {
+      if (ro >= 0 && ro < th && co >= 0 && co < TW) {
         incR[ro][co] = dvr;
         incT[ro][co] = dvt;
       }
     }
     __syncthreads();
-    k = 0;
-    for (int q = tid; q < HR32 * HC; q += SRC_TPB, k++) {
+#pragma unroll
+    for (int k = 0; k < SRC32_PER_THREAD; k++) {
+      const int q = tid + k * SRC_TPB;
+      if (q >= hr * HC) break;
       int r = q / HC, cc = q - r * HC;
       sA[r][cc] = regA[k];
       sB[r][cc] = regB[k];
@@ -791,10 +812,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
   // ---- S2: viscous stress (Kernels.cu:733-771) ----
   const float invdphi = (float)t.invdphi;
   const float onethird = 1.0f / 3.0f;
-  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+  for (int q = tid; q < hr * HC; q += SRC_TPB) {
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
-    if (r < 1 || r >= HR32 - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
+    if (r < 1 || r >= hr - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
     const float iRm = (float)t.invRmed[i], idRs = (float)t.invdiffRsup[i];
     const float vr0 = sA[r][cc], vr1 = sA[r + 1][cc];
     const float dvt = sB[r][cc + 1] - sB[r][cc];
@@ -817,14 +838,14 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
   __syncthreads();
 
   // ---- S3: viscous velocity update (Kernels.cu:118-150) ----
-  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+  for (int q = tid; q < hr * HC; q += SRC_TPB) {
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
-    if (r < 2 || r >= HR32 - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
+    if (r < 2 || r >= hr - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
     const float iRm = (float)t.invRmed[i];
     const int ro = r - HALO_R, co = cc - HALO_C;
-    const bool own = ro >= 0 && ro < TH32 && co >= 0 && co < TW;
-    if (i < nr - 1 && r < HR32 - 2) {
+    const bool own = ro >= 0 && ro < th && co >= 0 && co < TW;
+    if (i < nr - 1 && r < hr - 2) {
       float d = dtf * iRm *
                 (((float)t.Rsup[i] * sE[r + 1][cc] - (float)t.Rinf[i] * sE[r][cc]) * (float)t.invdiffRsup[i] +
                  (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5f * (sE[r][cc] + sE[r + 1][cc])) *
@@ -842,17 +863,17 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
   __syncthreads();
 
   // ---- S4: artificial pressure (Kernels.cu:409-434) ----
-  for (int q = tid; q < HR32 * HC; q += SRC_TPB) {
+  for (int q = tid; q < hr * HC; q += SRC_TPB) {
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
-    if (r < 3 || r >= HR32 - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
+    if (r < 3 || r >= hr - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
     sC[r][cc] = art_visc_q_f(sS[r][cc], sA[r + 1][cc] - sA[r][cc]);
     sD[r][cc] = art_visc_q_f(sS[r][cc], sB[r][cc + 1] - sB[r][cc]);
   }
   __syncthreads();
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
-  for (int q = tid; q < TH32 * TW; q += SRC_TPB) {
+  for (int q = tid; q < th * TW; q += SRC_TPB) {
     int ro = q / TW, co = q % TW;
     int r = ro + HALO_R, cc = co + HALO_C;
     int i = GI(r), j = j0 + co;
@@ -890,21 +911,28 @@ __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a) {
 #undef VREF
 }
 
+__global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a, int th) {
+  extern __shared__ float lds_src32[];
+  src32_tile(a, blockIdx.x, blockIdx.y, th, lds_src32);
+}
+
 // Radial sweep in FP32 (k_trad above): Q of each ring is formed in FP64,
 // its Work value, slopes, face values and fluxes in FP32; the flux
 // divergence is added to the FP64 Q.
-__global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) {
+// block index blk < NRAD: ring mean of that ring; otherwise one column
+// strip of one band of a.band rings
+__device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   const int nr = a.nr, ns = a.ns;
-  if ((int)blockIdx.x < nr) {
-    ring_mean_wave(a.vt + (long)blockIdx.x * ns, ns, a.vmed + blockIdx.x);
+  if (blk < nr) {
+    ring_mean_wave(a.vt + (long)blk * ns, ns, a.vmed + blk);
     return;
   }
-  const int bid = blockIdx.x - nr;
+  const int bid = blk - nr;
   const int j = (bid % a.nstrips) * TRAD_TPB + threadIdx.x;
   if (j >= ns) return;
   const int jp = (j + 1) % ns;
-  const int b0 = (bid / a.nstrips) * TRAD_BAND;
-  const int b1 = min(b0 + TRAD_BAND, nr);
+  const int b0 = (bid / a.nstrips) * a.band;
+  const int b1 = min(b0 + a.band, nr);
   const fg_dev_tables &t = a.t;
   const double dt = a.dt, OF = a.omega_frame;
   const float dtf = (float)dt;
@@ -982,6 +1010,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) {
   }
 }
 
+__global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
+
 // Azimuthal sweep in FP32 (azimuthal_sweep above); q stays FP64.
 template <int C, bool UNIFORM>
 __device__ __forceinline__ void azimuthal_sweep32(double (&q)[NQ][C], const float (&Vc)[C], float Vu, float *W,
@@ -1027,12 +1057,12 @@ __device__ __forceinline__ void azimuthal_sweep32(double (&q)[NQ][C], const floa
 #undef FOR_CELLS
 }
 
+// one ring; lds holds 2 * NSEC doubles
 template <int C>
-__global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds) {
   float *Wf = (float *)lds, *Df = Wf + a.ns;
   double *W = lds, *D = lds + a.ns;
-  const int ns = a.ns, i = blockIdx.x, tid = threadIdx.x;
+  const int ns = a.ns, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
   const double VMed = a.vmed[i];
@@ -1086,6 +1116,12 @@ __global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
     a.xrmm[base + j] = q[1][k];
   }
 #undef FOR_CELLS
+}
+
+template <int C>
+__global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
+  extern __shared__ double lds_taz32[];
+  taz32_ring<C>(a, blockIdx.x, lds_taz32);
 }
 
 // ----------------------------------------------------- entry points ----
@@ -1300,8 +1336,11 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     s3.indirect = sa.indirect;
     s3.evanescent = sa.evanescent;
     s3.corotating = sa.corotating;
-    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + TH32 - 1) / TH32), dim3(SRC_TPB), 0, c->stream,
-                       s3);
+    const int th = c->tile_src_th;
+    FG_HIP(hipFuncSetAttribute((const void *)k_src32, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)src32_lds_bytes(SRC32_MAX_TH)));
+    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC_TPB),
+                       src32_lds_bytes(th), c->stream, s3, th);
   } else {
     hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
   }
@@ -1323,8 +1362,9 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.ns = ns;
   ta.vmed = c->VMed;
   ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
+  ta.band = fp32 ? c->tile_trad_band : TRAD_BAND;
   prof_mark(c);
-  const dim3 gtrad(nr + ta.nstrips * ((nr + TRAD_BAND - 1) / TRAD_BAND));
+  const dim3 gtrad(nr + ta.nstrips * ((nr + ta.band - 1) / ta.band));
   if (fp32)
     hipLaunchKernelGGL(k_trad32, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
   else
@@ -1397,4 +1437,424 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   FG_LAUNCH_CHECK();
   prof_mark(c);
   return fused_reductions(c, DeltaT_next);
+}
+
+// ================================================== batched ensemble ====
+// Many independent disks (uneven grids, FP32 bulk mode) advanced one
+// substep together: each fused phase is ONE launch over a work list of
+// (disk, tile) items whose workgroups stride through the list, so small
+// and large disks share the machine.  Per substep the host writes every
+// disk's arguments (dt, planets, frame, active flag) into one pinned
+// block; a hipGraph captured once copies it to the device, runs the
+// phases and the reductions, and copies back each disk's CFL count and
+// star force.  The per-disk arithmetic is the single-disk FP32 code, so
+// a disk gives the same bits batched or alone.
+#include "fg_reduce.h"
+
+struct batch_disk {
+  src32_args src;
+  trad_args trad;
+  taz_args taz;
+  fin_args fin;
+  force_src force;
+  long ncell;      // NRAD * NSEC
+  int B;           // DeviceReduce blocks of the force sum
+  double *partial; // block sums (the context's reduction scratch)
+  double *DT1D, *Vmoy, *newDT;
+  const double *invRmed;
+  double coef, DeltaT;  // CFL coefficient and the next substep's DeltaT
+  int nr, active;
+};
+
+struct batch_result {
+  double fx, fy;
+  int cfl, pad;
+};
+
+#define BATCH_CAP_BLOCKS 4096  // workgroups of a persistent phase launch
+
+__global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *D, const int4 *items, int n) {
+  extern __shared__ float lds_bsrc[];
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int4 w = items[it];
+    if (D[w.x].active) src32_tile(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *D, const int2 *items, int n) {
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    if (D[w.x].active) trad32_block(D[w.x].trad, w.y);
+    __syncthreads();
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *D, const int2 *items, int n) {
+  extern __shared__ double lds_btaz[];
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    if (D[w.x].active) taz32_ring<C>(D[w.x].taz, w.y, lds_btaz);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_fin(const batch_disk *D, const int2 *items, int n) {
+  extern __shared__ double lds_bfin[];
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    if (D[w.x].active) fin_ring(D[w.x].fin, w.y, lds_bfin);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(RED_TPB) void k_batch_force(const batch_disk *D, const int2 *items, int n) {
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const batch_disk &d = D[w.x];
+    if (d.active) dreduce_block(d.force, d.ncell, d.B, w.y, d.partial);
+    __syncthreads();
+  }
+}
+
+// per disk: the force's final sum and the CFL count of the next substep
+__global__ __launch_bounds__(256) void k_batch_close(const batch_disk *D, batch_result *res, int ndisk) {
+  __shared__ double out[2];
+  for (int k = blockIdx.x; k < ndisk; k += gridDim.x) {
+    const batch_disk &d = D[k];
+    if (d.active) {
+      dreduce_final(d.partial, d.B, out, 1, 1);
+      __syncthreads();
+      cfl_final(&res[k].cfl, d.DT1D, d.Vmoy, d.newDT, d.invRmed, d.coef, d.DeltaT, d.nr);
+      if (threadIdx.x == 0) {
+        res[k].fx = out[0];
+        res[k].fy = out[1];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct fg_batch_dev {
+  int n;
+  std::vector<fg_ctx *> ctx;
+  hipStream_t stream;
+  batch_disk *h_disk, *d_disk;  // pinned host block and its device copy
+  batch_result *h_res, *d_res;
+  int4 *it_src;
+  int2 *it_trad, *it_taz, *it_fin, *it_force;
+  int n_src, n_trad, n_taz, n_fin, n_force;
+  int max_th, max_ns, taz_c;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int ready;  // items built and graph instantiated
+};
+
+static void batch_free(fg_batch_dev *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void *dev[] = {b->d_disk, b->d_res, b->it_src, b->it_trad, b->it_taz, b->it_fin, b->it_force};
+  for (void *q : dev)
+    if (q) (void)hipFree(q);
+  if (b->h_disk) (void)hipHostFree(b->h_disk);
+  if (b->h_res) (void)hipHostFree(b->h_res);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void fg_batch_dev_destroy(fg_batch_dev *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+}
+
+fg_status fg_batch_dev_create(fg_ctx *const *ctx, int n, fg_batch_dev **out) {
+  *out = nullptr;
+  if (n <= 0) {
+    fg_set_error("batch: no disks");
+    return FG_E_INVALID;
+  }
+  for (int k = 0; k < n; k++) {
+    if (!ctx[k] || ctx[k]->p.precision != FG_PREC_FP32) {
+      fg_set_error("batch: disk %d is not an FP32 bulk-mode context", k);
+      return FG_E_INVALID;
+    }
+    if (fg_status s = fused_supported(ctx[k])) return s;
+    if (fg_status s = fused_alloc(ctx[k])) return s;
+    if (ctx[k]->device != ctx[0]->device) {
+      fg_set_error("batch: all disks must live on one device");
+      return FG_E_INVALID;
+    }
+  }
+  fg_batch_dev *b = new fg_batch_dev();
+  b->n = n;
+  b->ctx.assign(ctx, ctx + n);
+  hipError_t e = hipStreamCreate(&b->stream);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_disk, n * sizeof(batch_disk), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_res, n * sizeof(batch_result), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->d_disk, n * sizeof(batch_disk));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->d_res, n * sizeof(batch_result));
+  if (e != hipSuccess) {
+    fg_set_error("batch: %s", hipGetErrorString(e));
+    batch_free(b);
+    return FG_E_HIP;
+  }
+  std::memset((void *)b->h_disk, 0, n * sizeof(batch_disk));
+  std::memset((void *)b->h_res, 0, n * sizeof(batch_result));
+  *out = b;
+  return FG_OK;
+}
+
+template <class T>
+static fg_status upload_items(const std::vector<T> &v, T **dev) {
+  FG_HIP(hipMalloc((void **)dev, (v.empty() ? 1 : v.size()) * sizeof(T)));
+  if (!v.empty()) FG_HIP(hipMemcpy(*dev, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FG_OK;
+}
+
+// work lists (largest disks first) and the captured graph of one substep
+static fg_status batch_build(fg_batch_dev *b) {
+  std::vector<int> order(b->n);
+  for (int k = 0; k < b->n; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->ctx[x]->S > b->ctx[y]->S; });
+  std::vector<int4> src;
+  std::vector<int2> trad, taz, fin, force;
+  b->max_th = 1;
+  b->max_ns = 1;
+  for (int k : order) {
+    const fg_ctx *c = b->ctx[k];
+    const int nr = c->nr, ns = c->ns, th = c->tile_src_th, band = c->tile_trad_band;
+    if (th < 1 || th > SRC32_MAX_TH || band < 1) {
+      fg_set_error("batch: disk %d has tile shape th=%d band=%d", k, th, band);
+      return FG_E_INVALID;
+    }
+    b->max_th = std::max(b->max_th, th);
+    b->max_ns = std::max(b->max_ns, ns);
+    for (int by = 0; by < (nr + 1 + th - 1) / th; by++)
+      for (int bx = 0; bx < (ns + TW - 1) / TW; bx++) src.push_back(make_int4(k, bx, by, th));
+    const int nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
+    const int nblk = nr + nstrips * ((nr + band - 1) / band);
+    for (int q = 0; q < nblk; q++) trad.push_back(make_int2(k, q));
+    for (int i = 0; i < nr; i++) {
+      taz.push_back(make_int2(k, i));
+      fin.push_back(make_int2(k, i));
+    }
+    const int B = red_blocks((long)nr * ns);
+    for (int q = 0; q < B; q++) force.push_back(make_int2(k, q));
+  }
+  b->taz_c = taz_cells(b->max_ns);
+  if (!b->taz_c) {
+    fg_set_error("batch: NSEC %d too large", b->max_ns);
+    return FG_E_UNSUPPORTED;
+  }
+  b->n_src = (int)src.size();
+  b->n_trad = (int)trad.size();
+  b->n_taz = (int)taz.size();
+  b->n_fin = (int)fin.size();
+  b->n_force = (int)force.size();
+  if (fg_status s = upload_items(src, &b->it_src)) return s;
+  if (fg_status s = upload_items(trad, &b->it_trad)) return s;
+  if (fg_status s = upload_items(taz, &b->it_taz)) return s;
+  if (fg_status s = upload_items(fin, &b->it_fin)) return s;
+  if (fg_status s = upload_items(force, &b->it_force)) return s;
+  FG_HIP(hipFuncSetAttribute((const void *)k_batch_src, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)src32_lds_bytes(SRC32_MAX_TH)));
+  // capture: arguments in, five phases, two reductions, results out
+  hipStream_t st = b->stream;
+  FG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  auto grid = [](int n) { return dim3(n < BATCH_CAP_BLOCKS ? (n > 0 ? n : 1) : BATCH_CAP_BLOCKS); };
+  (void)hipMemcpyAsync(b->d_disk, b->h_disk, b->n * sizeof(batch_disk), hipMemcpyHostToDevice, st);
+  hipLaunchKernelGGL(k_batch_src, grid(b->n_src), dim3(SRC_TPB), src32_lds_bytes(b->max_th), st, b->d_disk,
+                     b->it_src, b->n_src);
+  hipLaunchKernelGGL(k_batch_trad, grid(b->n_trad), dim3(TRAD_TPB), 0, st, b->d_disk, b->it_trad, b->n_trad);
+  const size_t shm_taz = 2 * (size_t)b->max_ns * sizeof(double);
+  const dim3 gt = grid(b->n_taz), bt(TAZ_TPB);
+  const batch_disk *D = b->d_disk;
+  const int2 *IT = b->it_taz;
+  const int NT = b->n_taz;
+  switch (b->taz_c) {
+    case 1: hipLaunchKernelGGL(k_batch_taz<1>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 2: hipLaunchKernelGGL(k_batch_taz<2>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 3: hipLaunchKernelGGL(k_batch_taz<3>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 4: hipLaunchKernelGGL(k_batch_taz<4>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 6: hipLaunchKernelGGL(k_batch_taz<6>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 8: hipLaunchKernelGGL(k_batch_taz<8>, gt, bt, shm_taz, st, D, IT, NT); break;
+    case 12: hipLaunchKernelGGL(k_batch_taz<12>, gt, bt, shm_taz, st, D, IT, NT); break;
+    default: hipLaunchKernelGGL(k_batch_taz<16>, gt, bt, shm_taz, st, D, IT, NT); break;
+  }
+  hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
+                     b->it_fin, b->n_fin);
+  hipLaunchKernelGGL(k_batch_force, grid(b->n_force), dim3(RED_TPB), 0, st, b->d_disk, b->it_force, b->n_force);
+  hipLaunchKernelGGL(k_batch_close, grid(b->n), dim3(256), 0, st, b->d_disk, b->d_res, b->n);
+  (void)hipMemcpyAsync(b->h_res, b->d_res, b->n * sizeof(batch_result), hipMemcpyDeviceToHost, st);
+  hipError_t ce = hipStreamEndCapture(st, &b->graph);
+  if (ce != hipSuccess) {
+    fg_set_error("batch graph capture: %s", hipGetErrorString(ce));
+    return FG_E_HIP;
+  }
+  FG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->ready = 1;
+  return FG_OK;
+}
+
+// arguments of disk k for its next substep (FP32 fused kernels); active=0
+// leaves the disk untouched this substep
+fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double omega_frame, double domega,
+                           const fg_planet_pot *planets, int nb, double ix, double iy, double DeltaT_next) {
+  if (!b || k < 0 || k >= b->n) return FG_E_INVALID;
+  batch_disk &d = b->h_disk[k];
+  d.active = active;
+  if (!active) return FG_OK;
+  if (nb < 0 || nb > FG_MAX_PLANETS || (nb > 0 && !planets)) {
+    fg_set_error("batch: planet count %d", nb);
+    return FG_E_INVALID;
+  }
+  fg_ctx *c = b->ctx[k];
+  const int nr = c->nr, ns = c->ns;
+  fused_bufs fb = bufs_of(c);
+  src32_args &s = d.src;
+  s.dens = c->f[FG_DENS];
+  s.vr = c->f[FG_VRAD];
+  s.vt = c->f[FG_VTHETA];
+  s.vref = c->Vmoy;
+  s.odens = fb.Bd;
+  s.ovr = fb.Bvr;
+  s.ovt = fb.Bvt;
+  s.opress = nullptr;
+  s.nu = c->nu;
+  s.t = c->t;
+  s.P.nb = nb;
+  for (int q = 0; q < nb; q++) {
+    s.P.x[q] = planets[q].x;
+    s.P.y[q] = planets[q].y;
+    s.P.m[q] = planets[q].mass;
+    s.P.smooth[q] = planets[q].smooth;
+    s.P.invd3[q] = planets[q].inv_d3;
+  }
+  s.P.ix = ix;
+  s.P.iy = iy;
+  s.dt = dt;
+  s.omega_frame = omega_frame;
+  s.domega = domega;
+  s.nr = nr;
+  s.ns = ns;
+  s.indirect = c->p.indirect_term;
+  s.evanescent = c->p.evanescent;
+  s.corotating = (domega != 0.0) ? 1 : 0;
+  trad_args &ta = d.trad;
+  ta.dens = fb.Bd;
+  ta.vr = fb.Bvr;
+  ta.vt = fb.Bvt;
+  ta.out[0] = fb.Rrmp;
+  ta.out[1] = fb.Rrmm;
+  ta.out[2] = fb.Rtmp;
+  ta.out[3] = fb.Rtmm;
+  ta.out[4] = fb.Rd;
+  ta.vmed = c->VMed;
+  ta.t = c->t;
+  ta.dt = dt;
+  ta.omega_frame = omega_frame;
+  ta.nr = nr;
+  ta.ns = ns;
+  ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
+  ta.band = c->tile_trad_band;
+  taz_args &za = d.taz;
+  za.vt = fb.Bvt;
+  za.vmed = c->VMed;
+  za.q[0] = fb.Rrmp;
+  za.q[1] = fb.Rrmm;
+  za.q[2] = fb.Rtmp;
+  za.q[3] = fb.Rtmm;
+  za.q[4] = fb.Rd;
+  za.xd = fb.Xd;
+  za.xvt = fb.Xvt;
+  za.xrmp = fb.Xrmp;
+  za.xrmm = fb.Xrmm;
+  za.nshift = c->Nshift;
+  za.t = c->t;
+  za.dt = dt;
+  za.omega_frame = omega_frame;
+  za.nr = nr;
+  za.ns = ns;
+  za.fast = c->p.fast_transport;
+  fin_args &fa = d.fin;
+  fa.xd = fb.Xd;
+  fa.xvt = fb.Xvt;
+  fa.xrmp = fb.Xrmp;
+  fa.xrmm = fb.Xrmm;
+  fa.press = fb.P;
+  fa.dens = c->f[FG_DENS];
+  fa.vr = c->f[FG_VRAD];
+  fa.vt = c->f[FG_VTHETA];
+  fa.temp = nullptr;
+  fa.vmoy = c->Vmoy;
+  fa.newdt = c->newDT;
+  fa.nu = c->nu;
+  fa.t = c->t;
+  fa.dt = dt;
+  fa.omega_frame = omega_frame;
+  fa.nr = nr;
+  fa.ns = ns;
+  fa.evanescent = c->p.evanescent;
+  fa.fast = c->p.fast_transport;
+  d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
+  d.ncell = (long)nr * ns;
+  d.B = red_blocks(d.ncell);
+  d.partial = c->red_partial;
+  d.DT1D = c->DT1D;
+  d.Vmoy = c->Vmoy;
+  d.newDT = c->newDT;
+  d.invRmed = c->t.invRmed;
+  d.coef = c->t.dt1d_coef;
+  d.DeltaT = DeltaT_next;
+  d.nr = nr;
+  return FG_OK;
+}
+
+// run one substep of every active disk (asynchronous)
+fg_status fg_batch_dev_launch(fg_batch_dev *b) {
+  if (!b->ready)
+    if (fg_status s = batch_build(b)) return s;
+  FG_HIP(hipGraphLaunch(b->exec, b->stream));
+  return FG_OK;
+}
+
+// wait for the last launch; results of disk k
+fg_status fg_batch_dev_wait(fg_batch_dev *b) {
+  FG_HIP(hipStreamSynchronize(b->stream));
+  return FG_OK;
+}
+
+void fg_batch_dev_result(const fg_batch_dev *b, int k, int *cfl, double *force2) {
+  *cfl = b->h_res[k].cfl;
+  force2[0] = b->h_res[k].fx;
+  force2[1] = b->h_res[k].fy;
+}
+
+// after a tile-shape change the work lists and the graph are rebuilt
+void fg_batch_dev_invalidate(fg_batch_dev *b) {
+  if (!b || !b->ready) return;
+  (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (void **q : {(void **)&b->it_src, (void **)&b->it_trad, (void **)&b->it_taz, (void **)&b->it_fin,
+                   (void **)&b->it_force}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
+  b->ready = 0;
+}
+
+extern "C" fg_status fg_set_tiles(fg_ctx *c, int src_rows, int trad_band) {
+  if (!c) return FG_E_INVALID;
+  if (src_rows < 1 || src_rows > SRC32_MAX_TH || trad_band < 1 || trad_band > 4096) {
+    fg_set_error("fg_set_tiles: rows %d (1..%d), band %d", src_rows, SRC32_MAX_TH, trad_band);
+    return FG_E_INVALID;
+  }
+  c->tile_src_th = src_rows;
+  c->tile_trad_band = trad_band;
+  return FG_OK;
 }

@@ -291,6 +291,7 @@ struct fg_sim {
   int in_dt = 0, last_cfl = 0;
   double IndirectX = 0.0, IndirectY = 0.0;
   int fused_active = 0;   // substeps run through fg_step_fused
+  long nsub = 0;          // substeps taken since creation
   int results_ready = 0;  // fused CFL count / star force are current
 };
 
@@ -983,23 +984,31 @@ extern "C" fg_status fg_sim_destroy(fg_sim *s) {
 
 // AlgoGas's loop body on the fused path: the same host-side sequence, the
 // GPU part in one fg_step_fused call whose reductions already hold the
-// next substep's CFL count and star force.
-static fg_status fused_substep(fg_sim *s, int *closed) {
+// next substep's CFL count and star force.  Split in three so that a
+// batch of disks can run the GPU part of all of them in one launch:
+//   fused_open   start of a DT sub-cycle, mass taper
+//   fused_plan   dt from the CFL count, indirect term from the star force,
+//                planets (disk kick, RK5, frame) and their potential terms
+//   fused_close  time bookkeeping after the GPU part
+struct fused_launch {
+  double dt, domega, DeltaT_next;
+  fg_planet_pot pl[FG_MAX_PLANETS];
+  int nb, done;
+};
+
+static void fused_open(fg_sim *s) {
   const Deck &d = s->d;
-  fg_ctx *c = s->ctx;
   if (!s->in_dt) {
     s->dtemp = 0.0;
     s->in_dt = 1;
   }
   s->MassTaper = s->PhysicalTime / (d.MASSTAPER * 2.0 * M_PI);
   s->MassTaper = (s->MassTaper > 1.0 ? 1.0 : pow(sin(s->MassTaper * M_PI / 2.0), 2.0));
-  if (!s->results_ready) {
-    CHECK(fg_fused_prepare(c, d.DT - s->dtemp));
-    s->results_ready = 1;
-  }
-  int cfl = 1;
-  double force[2];
-  CHECK(fg_fused_results(c, &cfl, force, nullptr));
+}
+
+static fg_status fused_plan(fg_sim *s, int cfl, const double force[2], fused_launch &L) {
+  const Deck &d = s->d;
+  fg_ctx *c = s->ctx;
   s->last_cfl = cfl;
   const double dt = (d.DT - s->dtemp) / (double)cfl;
   s->dt = dt;
@@ -1007,8 +1016,7 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
   if (d.Corotating) psys_info(s, 1);
   s->IndirectX = d.Indirect_Term ? -force[0] : 0.0;
   s->IndirectY = d.Indirect_Term ? -force[1] : 0.0;
-  fg_planet_pot pl[FG_MAX_PLANETS];
-  const int nb = potential_planets(s, pl);
+  L.nb = potential_planets(s, L.pl);
   Planets &p = s->ps;
   for (int k = 0; k < p.nb; k++) {  // AdvanceSystemFromDisk, Pframeforce.cu:97-121
     if (!p.feeldisk[k]) continue;
@@ -1028,14 +1036,37 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
     s->OmegaFrame = OmegaNew;
   }
   rotate_psys(s, s->OmegaFrame * dt);
-  const int done = !(s->dtemp < 0.999999999 * d.DT);
-  const double DeltaT_next = done ? d.DT : d.DT - s->dtemp;
-  CHECK(fg_step_fused(c, dt, s->OmegaFrame, domega, pl, nb, s->IndirectX, s->IndirectY, DeltaT_next,
-                      d.Write_Temperature));
-  s->PhysicalTime += dt;
+  L.done = !(s->dtemp < 0.999999999 * d.DT);
+  L.DeltaT_next = L.done ? d.DT : d.DT - s->dtemp;
+  L.dt = dt;
+  L.domega = domega;
+  return FG_OK;
+}
+
+static void fused_close(fg_sim *s, const fused_launch &L, int *closed) {
+  s->nsub++;
+  s->PhysicalTime += L.dt;
   s->prof.time = s->PhysicalTime;
-  if (done) s->in_dt = 0;
-  if (closed) *closed = done;
+  if (L.done) s->in_dt = 0;
+  if (closed) *closed = L.done;
+}
+
+static fg_status fused_substep(fg_sim *s, int *closed) {
+  const Deck &d = s->d;
+  fg_ctx *c = s->ctx;
+  fused_open(s);
+  if (!s->results_ready) {
+    CHECK(fg_fused_prepare(c, d.DT - s->dtemp));
+    s->results_ready = 1;
+  }
+  int cfl = 1;
+  double force[2];
+  CHECK(fg_fused_results(c, &cfl, force, nullptr));
+  fused_launch L;
+  CHECK(fused_plan(s, cfl, force, L));
+  CHECK(fg_step_fused(c, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY, L.DeltaT_next,
+                      d.Write_Temperature));
+  fused_close(s, L, closed);
   return FG_OK;
 }
 
@@ -1101,6 +1132,7 @@ extern "C" fg_status fg_sim_substep(fg_sim *s, int *closed) {
     CHECK(fg_boundary(c, dt, s->OmegaFrame));
     CHECK(fg_temperature(c));
   }
+  s->nsub++;
   s->PhysicalTime += dt;
   s->prof.time = s->PhysicalTime;
   int done = !(s->dtemp < 0.999999999 * d.DT);
@@ -1206,7 +1238,7 @@ extern "C" fg_status fg_sim_get(fg_sim *s, const char *name, double *v) {
              {"NSEC", (double)s->ns},           {"last_cfl", (double)s->last_cfl},
              {"in_dt", (double)s->in_dt},       {"IndirectTerm.x", s->IndirectX},
              {"IndirectTerm.y", s->IndirectY},  {"HillRadius", s->HillRadius},
-             {"Adiabatic", (double)s->d.Adiabatic}, {"SelfGravity", (double)s->d.SelfGravity},
+             {"Adiabatic", (double)s->d.Adiabatic}, {"substeps", (double)s->nsub}, {"SelfGravity", (double)s->d.SelfGravity},
              {"fused_active", (double)s->fused_active}};
   for (auto &e : tab)
     if (strcmp(e.k, name) == 0) {
@@ -1251,4 +1283,140 @@ extern "C" fg_status fg_sim_ring(fg_sim *s, const char *name, double *out) {
   }
   memcpy(out, v->data(), v->size() * sizeof(double));
   return FG_OK;
+}
+
+// ------------------------------------------------------ ensemble batch --
+// Independent disks (each its own deck, FP32 bulk mode) advanced together:
+// per batch substep every disk short of the target time takes one substep
+// of its own sub-cycle (its own CFL-limited dt); the host part of each
+// (dt, planets, frame) runs here in FP64 and the GPU part of all of them
+// is one captured graph (fg_fused.hip).  A disk that has reached the
+// target idles.
+struct fg_batch_dev;
+fg_status fg_batch_dev_create(fg_ctx *const *ctx, int n, fg_batch_dev **out);
+void fg_batch_dev_destroy(fg_batch_dev *b);
+fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double omega_frame, double domega,
+                           const fg_planet_pot *planets, int nb, double ix, double iy, double DeltaT_next);
+fg_status fg_batch_dev_launch(fg_batch_dev *b);
+fg_status fg_batch_dev_wait(fg_batch_dev *b);
+void fg_batch_dev_result(const fg_batch_dev *b, int k, int *cfl, double *force2);
+void fg_batch_dev_invalidate(fg_batch_dev *b);
+
+struct fg_batch {
+  std::vector<fg_sim *> sims;
+  fg_batch_dev *dev = nullptr;
+  std::vector<int> cfl;
+  std::vector<double> force;
+  std::vector<fused_launch> L;
+  std::vector<char> active;
+  int started = 0;
+};
+
+extern "C" fg_status fg_batch_destroy(fg_batch *b) {
+  if (!b) return FG_OK;
+  fg_batch_dev_destroy(b->dev);
+  for (fg_sim *s : b->sims) fg_sim_destroy(s);
+  delete b;
+  return FG_OK;
+}
+
+extern "C" fg_status fg_batch_create(int n, const char *const *parfiles, const fg_sim_options *opt,
+                                     fg_batch **out) {
+  if (!out || !parfiles || n <= 0) {
+    host_error("fg_batch_create: bad argument");
+    return FG_E_INVALID;
+  }
+  *out = nullptr;
+  fg_sim_options o = opt ? *opt : fg_sim_options{};
+  o.precision = FG_PREC_FP32;
+  o.fused = 1;
+  fg_batch *b = new fg_batch();
+  for (int k = 0; k < n; k++) {
+    fg_sim *s = nullptr;
+    fg_status st = fg_sim_create(parfiles[k], &o, &s);
+    if (st) {
+      fg_batch_destroy(b);
+      return st;
+    }
+    b->sims.push_back(s);
+  }
+  std::vector<fg_ctx *> ctx(n);
+  for (int k = 0; k < n; k++) ctx[k] = b->sims[k]->ctx;
+  fg_status st = fg_batch_dev_create(ctx.data(), n, &b->dev);
+  if (st) {
+    fg_batch_destroy(b);
+    return st;
+  }
+  b->cfl.assign(n, 1);
+  b->force.assign(2 * (size_t)n, 0.0);
+  b->L.resize(n);
+  b->active.assign(n, 0);
+  *out = b;
+  return FG_OK;
+}
+
+extern "C" fg_status fg_batch_set_tiles(fg_batch *b, int k, int src_rows, int trad_band) {
+  if (!b || k < 0 || k >= (int)b->sims.size()) return FG_E_INVALID;
+  CHECK(fg_set_tiles(b->sims[k]->ctx, src_rows, trad_band));
+  fg_batch_dev_invalidate(b->dev);
+  return FG_OK;
+}
+
+extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) {
+  if (!b) return FG_E_INVALID;
+  const int n = (int)b->sims.size();
+  if (!b->started) {  // CFL count and star force of every initial state
+    for (int k = 0; k < n; k++) {
+      fg_sim *s = b->sims[k];
+      fused_open(s);
+      CHECK(fg_fused_prepare(s->ctx, s->d.DT - s->dtemp));
+      CHECK(fg_fused_results(s->ctx, &b->cfl[k], &b->force[2 * k], nullptr));
+    }
+    b->started = 1;
+  }
+  int na = 0;
+  for (int k = 0; k < n; k++) {
+    fg_sim *s = b->sims[k];
+    // a disk stops at the first DT boundary at or after t_end
+    const bool act = s->in_dt || s->PhysicalTime < t_end - 1e-9 * s->d.DT;
+    b->active[k] = act;
+    if (!act) {
+      CHECK(fg_batch_dev_set(b->dev, k, 0, 0, 0, 0, nullptr, 0, 0, 0, 0));
+      continue;
+    }
+    na++;
+    fused_open(s);
+    fused_launch &L = b->L[k];
+    CHECK(fused_plan(s, b->cfl[k], &b->force[2 * k], L));
+    CHECK(fg_batch_dev_set(b->dev, k, 1, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY,
+                           L.DeltaT_next));
+  }
+  if (n_active) *n_active = na;
+  if (!na) return FG_OK;
+  CHECK(fg_batch_dev_launch(b->dev));
+  CHECK(fg_batch_dev_wait(b->dev));
+  for (int k = 0; k < n; k++) {
+    if (!b->active[k]) continue;
+    fg_batch_dev_result(b->dev, k, &b->cfl[k], &b->force[2 * k]);
+    fused_close(b->sims[k], b->L[k], nullptr);
+    b->sims[k]->results_ready = 0;  // single-disk stepping re-derives them
+  }
+  return FG_OK;
+}
+
+extern "C" fg_status fg_batch_run(fg_batch *b, double t_end, long *substeps) {
+  if (!b) return FG_E_INVALID;
+  long count = 0;
+  int na = 1;
+  while (na) {
+    CHECK(fg_batch_substep(b, t_end, &na));
+    if (na) count++;
+  }
+  if (substeps) *substeps = count;
+  return FG_OK;
+}
+
+extern "C" fg_sim *fg_batch_sim(fg_batch *b, int k) {
+  if (!b || k < 0 || k >= (int)b->sims.size()) return nullptr;
+  return b->sims[k];
 }

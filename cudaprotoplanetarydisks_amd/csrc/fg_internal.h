@@ -100,6 +100,9 @@ struct fg_ctx {
   int prof_nev, prof_pending;
   double prof_ms[16];
   long prof_calls[16];
+  // tile shapes of the FP32 fused kernels (rings per k_src tile, rings
+  // per radial-sweep band), set by fg_set_tiles
+  int tile_src_th, tile_trad_band;
   // fused-path scratch
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results

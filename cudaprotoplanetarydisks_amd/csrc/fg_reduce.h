@@ -1,0 +1,143 @@
+// DeviceReduce-order sums (Kernels.cu:253-394) and the disk-gravity
+// summand of ComputeAccel (SideEuler.cu:135, Force.cu:102) as device
+// functions, shared by the single-context kernels (fg_reduce.hip) and the
+// batched ensemble kernels (fg_fused.hip).
+//
+// DeviceReduce launches B = min(pow2(n)/32, 1024) blocks of 32 threads.
+// Thread t of block b accumulates, for k = 0, 1, ..., the elements
+// e = 64*b + k*64*B + t and e + 32 (while e < n), then the 32 partials of
+// a block meet in a shuffle-down tree (offsets 16, 8, 4, 2, 1) and the
+// host adds the B block results in order.  Here one 256-thread workgroup
+// replays one such block: its four waves evaluate the summands of CH
+// consecutive k into LDS, then wave 0 (lanes 0-31 for x, 32-63 for y)
+// adds them in the original order, and a width-32 shuffle tree finishes
+// the block.  The final sequential sum over blocks runs in one lane.
+// The result is bit-identical to the reference's reduction order while
+// the summands are computed with full occupancy.
+#pragma once
+
+#include "fg_internal.h"
+
+#define RED_TPB 256
+#define RED_CH 16
+
+static inline int red_blocks(long n) {
+  long p = 1;
+  while (p < n) p <<= 1;
+  long b = (p + 31) / 32;
+  return (int)(b > FG_RED_BLOCKS ? FG_RED_BLOCKS : b);
+}
+
+// gravity of cell e on a point mass at (x, y), Kernels.cu:703-727 with
+// G = 1 and hill_cut = 1 (a = rh = 0 in ComputeAccel)
+struct force_src {
+  const double *dens, *Surf, *Rmed, *cosc, *sinc;
+  double x, y, rs;
+  int ns;
+  __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
+    int i = (int)(e / ns);
+    int j = (int)(e - (long)i * ns);
+    double cellmass = Surf[i] * dens[e];
+    double dx = Rmed[i] * cosc[j] - x;
+    double dy = Rmed[i] * sinc[j] - y;
+    double d2 = dx * dx + dy * dy;
+    double dist2 = d2 + rs * rs;
+    double distance = sqrt(dist2);
+    double InvDist3 = 1.0 / dist2 / distance;
+    fx = cellmass * dx * InvDist3;
+    fy = cellmass * dy * InvDist3;
+  }
+};
+
+struct array_src {
+  const double *a, *b;  // b may be NULL
+  __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
+    fx = a[e];
+    fy = b ? b[e] : 0.0;
+  }
+};
+
+// block blk of B (a RED_TPB-thread workgroup); partial[blk] and
+// partial[B + blk] receive the x and y block sums
+template <class Src>
+__device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int blk, double *partial) {
+  __shared__ double vx[RED_CH][64];
+  __shared__ double vy[RED_CH][64];
+  const long G = 64L * B;
+  const long b64 = 64L * blk;
+  double acc = 0.0;
+  for (long k0 = 0; b64 + k0 * G < n; k0 += RED_CH) {
+    for (int c = threadIdx.x; c < RED_CH * 64; c += RED_TPB) {
+      int kk = c >> 6, lane = c & 63;
+      long e = b64 + (k0 + kk) * G + lane;
+      double fx = 0.0, fy = 0.0;
+      if (e < n) src(e, fx, fy);
+      vx[kk][lane] = fx;
+      vy[kk][lane] = fy;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int t = threadIdx.x & 31;
+      const bool isy = threadIdx.x >= 32;
+      for (int kk = 0; kk < RED_CH; kk++) {
+        long e = b64 + (k0 + kk) * G + t;
+        if (e < n) {
+          acc += isy ? vy[kk][t] : vx[kk][t];
+          if (e + 32 < n) acc += isy ? vy[kk][t + 32] : vx[kk][t + 32];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 64) {
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_down(acc, off, 32);
+    if (threadIdx.x == 0) partial[blk] = acc;
+    if (threadIdx.x == 32) partial[B + blk] = acc;
+  }
+}
+
+// host loop of DeviceReduce (Kernels.cu:385-390).  With add_inner, the
+// result is fx_inner + fx_outer of SideEuler.cu:144 where the inner sum
+// is the DeviceReduce of an all-zero array (+0.0).
+__device__ __forceinline__ void dreduce_final(const double *partial, int B, double *out, int two, int add_inner) {
+  __shared__ double p[2 * FG_RED_BLOCKS];
+  const int nv = two ? 2 : 1;
+  // stage the block sums in LDS with every lane, then add them in order
+  for (int k = threadIdx.x; k < nv * B; k += blockDim.x) p[k] = partial[(k / B) * B + (k % B)];
+  __syncthreads();
+  int lane = threadIdx.x;
+  if (lane >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < B; b++) s += p[lane * B + b];
+  if (add_inner) s = 0.0 + s;
+  out[lane] = s;
+}
+
+
+// ConditionCFLKernel2D3 (Kernels.cu:1337-1360) and the host side of
+// ConditionCFL (SourceEuler.cu:520-546): min of the ring minima and the
+// 1D shear limits, then the sub-cycle count ceil(DeltaT / dt)
+__device__ __forceinline__ void cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt,
+                                          const double *invRmed, double coef, double DeltaT, int nr) {
+  __shared__ double red[256];
+  double best = 1e300;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    if (i >= 1 && newdt[i] < best) best = newdt[i];
+    if (i < nr - 1) {
+      double d = coef / fabs(vmoy[i] * invRmed[i] - vmoy[i + 1] * invRmed[i + 1]);
+      dt1d[i] = d;
+      if (d < best) best = d;
+    }
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s && red[threadIdx.x + s] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double newdt_ = red[0];
+    if (DeltaT < newdt_) newdt_ = DeltaT;
+    cfl[0] = (int)(ceil(DeltaT / newdt_));
+  }
+}

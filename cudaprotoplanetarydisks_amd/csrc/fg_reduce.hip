@@ -1,110 +1,15 @@
-// Sums in DeviceReduce order (Kernels.cu:253-394) and the disk-gravity
-// reduction of ComputeAccel (SideEuler.cu:135, Force.cu:102).
-//
-// DeviceReduce launches B = min(pow2(n)/32, 1024) blocks of 32 threads.
-// Thread t of block b accumulates, for k = 0, 1, ..., the elements
-// e = 64*b + k*64*B + t and e + 32 (while e < n), then the 32 partials of
-// a block meet in a shuffle-down tree (offsets 16, 8, 4, 2, 1) and the
-// host adds the B block results in order.  Here one 256-thread workgroup
-// replays one such block: its four waves evaluate the summands of CH
-// consecutive k into LDS, then wave 0 (lanes 0-31 for x, 32-63 for y)
-// adds them in the original order, and a width-32 shuffle tree finishes
-// the block.  The final sequential sum over blocks runs in one lane.
-// The result is bit-identical to the reference's reduction order while
-// the summands are computed with full occupancy.
-#include "fg_internal.h"
-
-#define RED_TPB 256
-#define RED_CH 16
-
-static int red_blocks(long n) {
-  long p = 1;
-  while (p < n) p <<= 1;
-  long b = (p + 31) / 32;
-  return (int)(b > FG_RED_BLOCKS ? FG_RED_BLOCKS : b);
-}
-
-// gravity of cell e on a point mass at (x, y), Kernels.cu:703-727 with
-// G = 1 and hill_cut = 1 (a = rh = 0 in ComputeAccel)
-struct force_src {
-  const double *dens, *Surf, *Rmed, *cosc, *sinc;
-  double x, y, rs;
-  int ns;
-  __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
-    int i = (int)(e / ns);
-    int j = (int)(e - (long)i * ns);
-    double cellmass = Surf[i] * dens[e];
-    double dx = Rmed[i] * cosc[j] - x;
-    double dy = Rmed[i] * sinc[j] - y;
-    double d2 = dx * dx + dy * dy;
-    double dist2 = d2 + rs * rs;
-    double distance = sqrt(dist2);
-    double InvDist3 = 1.0 / dist2 / distance;
-    fx = cellmass * dx * InvDist3;
-    fy = cellmass * dy * InvDist3;
-  }
-};
-
-struct array_src {
-  const double *a, *b;  // b may be NULL
-  __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
-    fx = a[e];
-    fy = b ? b[e] : 0.0;
-  }
-};
+// Single-context reductions: DeviceReduce-order sums and the disk force
+// (the device functions live in fg_reduce.h), and the non-reflecting
+// boundary, whose ring means are such sums.
+#include "fg_reduce.h"
 
 template <class Src>
 __global__ __launch_bounds__(RED_TPB) void k_dreduce_partial(Src src, long n, int B, double *partial) {
-  __shared__ double vx[RED_CH][64];
-  __shared__ double vy[RED_CH][64];
-  const long G = 64L * B;
-  const long b64 = 64L * blockIdx.x;
-  double acc = 0.0;
-  for (long k0 = 0; b64 + k0 * G < n; k0 += RED_CH) {
-    for (int c = threadIdx.x; c < RED_CH * 64; c += RED_TPB) {
-      int kk = c >> 6, lane = c & 63;
-      long e = b64 + (k0 + kk) * G + lane;
-      double fx = 0.0, fy = 0.0;
-      if (e < n) src(e, fx, fy);
-      vx[kk][lane] = fx;
-      vy[kk][lane] = fy;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const int t = threadIdx.x & 31;
-      const bool isy = threadIdx.x >= 32;
-      for (int kk = 0; kk < RED_CH; kk++) {
-        long e = b64 + (k0 + kk) * G + t;
-        if (e < n) {
-          acc += isy ? vy[kk][t] : vx[kk][t];
-          if (e + 32 < n) acc += isy ? vy[kk][t + 32] : vx[kk][t + 32];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 64) {
-    for (int off = 16; off > 0; off >>= 1) acc += __shfl_down(acc, off, 32);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-    if (threadIdx.x == 32) partial[B + blockIdx.x] = acc;
-  }
+  dreduce_block(src, n, B, blockIdx.x, partial);
 }
 
-// host loop of DeviceReduce (Kernels.cu:385-390).  With add_inner, the
-// result is fx_inner + fx_outer of SideEuler.cu:144 where the inner sum
-// is the DeviceReduce of an all-zero array (+0.0).
 __global__ void k_dreduce_final(const double *partial, int B, double *out, int two, int add_inner) {
-  __shared__ double p[2 * FG_RED_BLOCKS];
-  const int nv = two ? 2 : 1;
-  // stage the block sums in LDS with every lane, then add them in order
-  for (int k = threadIdx.x; k < nv * B; k += blockDim.x) p[k] = partial[(k / B) * B + (k % B)];
-  __syncthreads();
-  int lane = threadIdx.x;
-  if (lane >= nv) return;
-  double s = 0.0;
-  for (int b = 0; b < B; b++) s += p[lane * B + b];
-  if (add_inner) s = 0.0 + s;
-  out[lane] = s;
+  dreduce_final(partial, B, out, two, add_inner);
 }
 
 template <class Src>

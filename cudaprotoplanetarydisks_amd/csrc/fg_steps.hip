@@ -7,6 +7,7 @@
 // major, sectors contiguous, so a wavefront reads 64 consecutive doubles).
 #include "fg_cell.h"
 #include "fg_internal.h"
+#include "fg_reduce.h"
 
 #define TPB 256
 
@@ -476,27 +477,7 @@ __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const 
 // ConditionCFLKernel2D2 (DT1D) + 2D3, Kernels.cu:1331-1359
 __global__ void k_cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt,
                             const double *invRmed, double coef, double DeltaT, int nr) {
-  __shared__ double red[256];
-  double best = 1e300;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    if (i >= 1 && newdt[i] < best) best = newdt[i];
-    if (i < nr - 1) {
-      double d = coef / fabs(vmoy[i] * invRmed[i] - vmoy[i + 1] * invRmed[i + 1]);
-      dt1d[i] = d;
-      if (d < best) best = d;
-    }
-  }
-  red[threadIdx.x] = best;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s && red[threadIdx.x + s] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double newdt_ = red[0];
-    if (DeltaT < newdt_) newdt_ = DeltaT;
-    cfl[0] = (int)(ceil(DeltaT / newdt_));
-  }
+  cfl_final(cfl, dt1d, vmoy, newdt, invRmed, coef, DeltaT, nr);
 }
 
 // ----------------------------------------------------- entry points --

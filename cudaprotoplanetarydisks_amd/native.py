@@ -80,6 +80,7 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "fg_fused_prepare": (ctypes.c_int, [_vp, _d]),
     "fg_fused_results": (ctypes.c_int, [_vp, _pi, _pd, _pd]),
+    "fg_set_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "fg_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fg_profile_read": (ctypes.c_int, [_vp, _pd, ctypes.POINTER(ctypes.c_long), ctypes.c_int]),
     "fg_sim_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FgSimOptions), ctypes.POINTER(_vp)]),
@@ -95,6 +96,13 @@ SIGNATURES = {
     "fg_sim_planet": (ctypes.c_int, [_vp, ctypes.c_int, _pd]),
     "fg_sim_ctx": (_vp, [_vp]),
     "fg_sim_ring": (ctypes.c_int, [_vp, ctypes.c_char_p, _pd]),
+    "fg_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(FgSimOptions),
+                                       ctypes.POINTER(_vp)]),
+    "fg_batch_destroy": (ctypes.c_int, [_vp]),
+    "fg_batch_set_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fg_batch_substep": (ctypes.c_int, [_vp, _d, _pi]),
+    "fg_batch_run": (ctypes.c_int, [_vp, _d, ctypes.POINTER(ctypes.c_long)]),
+    "fg_batch_sim": (_vp, [_vp, ctypes.c_int]),
 }
 
 

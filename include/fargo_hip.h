@@ -205,6 +205,12 @@ fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domeg
 /* CFL count and star force of the current state (fused reductions) */
 fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
 fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);
+/* Tile shapes of the FP32 fused kernels: rings per source tile (1..32)
+ * and rings per radial-sweep band.  Results do not depend on them; the
+ * autotuner (cudaprotoplanetarydisks_amd/autotune.py) picks them per grid.
+ * No reference counterpart (the reference's launch shape is fixed,
+ * Main.cu:204-226). */
+fg_status fg_set_tiles(fg_ctx *ctx, int src_rows, int trad_band);
 
 /* ---- tracing ---------------------------------------------------------- */
 /* Kernel timing of the fused path with HIP events on the context's
@@ -254,6 +260,26 @@ fg_ctx *fg_sim_ctx(fg_sim *sim);
 /* 1D grid of the run (NRAD+1 entries): "Radii", "Rinf", "Rsup", "Rmed",
  * "Surf", "SigmaMed", "EnergyMed" */
 fg_status fg_sim_ring(fg_sim *sim, const char *name, double *out);
+
+/* ---- ensemble: many independent disks in one launch per phase ------ */
+/* A parameter-sweep ensemble of FP32 bulk-mode disks (uneven grids).  Per
+ * batch substep every disk that has not reached t_end takes one substep
+ * of its own AlgoGas sub-cycle (SourceEuler.cu:222-305) with its own dt;
+ * the GPU part of all disks is one replay of a captured hipGraph (each
+ * fused phase one launch over a (disk, tile) work list).  A disk stops at
+ * the first DT boundary at or after t_end.  Each disk gives the same
+ * result as the same deck run alone with precision FG_PREC_FP32.  The
+ * reference runs one disk per process (Main.cu:89); no counterpart. */
+typedef struct fg_batch fg_batch;
+fg_status fg_batch_create(int n, const char *const *parfiles, const fg_sim_options *opt, fg_batch **out);
+fg_status fg_batch_destroy(fg_batch *b);
+fg_status fg_batch_set_tiles(fg_batch *b, int k, int src_rows, int trad_band);
+/* one batch substep; *n_active = disks that moved (0: all reached t_end) */
+fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active);
+/* batch substeps until every disk reached t_end */
+fg_status fg_batch_run(fg_batch *b, double t_end, long *substeps);
+/* disk k (state, scalars, planets through the fg_sim_* calls) */
+fg_sim *fg_batch_sim(fg_batch *b, int k);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,83 @@
+"""Batched ensemble (fg_batch_*): uneven grids in one launch per phase,
+replayed from a captured hipGraph.  Each disk of a batch must give the
+same bits as the same deck run alone in the FP32 bulk mode, whatever the
+tile shapes and whatever the other disks are doing (idle or not)."""
+import math
+
+import numpy as np
+import pytest
+
+from cudaprotoplanetarydisks_amd import Simulation
+from cudaprotoplanetarydisks_amd.ensemble import Ensemble, draw, write_disk_deck
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [dict(nrad=24, nsec=72, q=1e-3, alpha=1e-3, h=0.05, slope=0.0, flaring=0.0),
+         dict(nrad=40, nsec=120, q=3e-4, alpha=3e-3, h=0.04, slope=-1.0, flaring=0.0),
+         dict(nrad=32, nsec=96, q=2e-3, alpha=2e-4, h=0.07, slope=-0.5, flaring=0.1)]
+
+
+def decks(tmp_path, params):
+    return [write_disk_deck(str(tmp_path), k, p) for k, p in enumerate(params)]
+
+
+def alone(deck, t_end):
+    with Simulation(deck, precision="fp32") as s:
+        while s.get("PhysicalTime") < t_end - 1e-9 * s.get("DT"):
+            s.algogas()
+        return s.state(), s.get("PhysicalTime")
+
+
+def test_batch_equals_single_disk_runs(tmp_path):
+    ds = decks(tmp_path, SMALL)
+    t_end = 4 * 0.314159265359
+    with Ensemble(ds, tiles={}) as ens:
+        steps = ens.run(t_end)
+        assert steps > 0
+        for k, d in enumerate(ds):
+            ref, t = alone(d, t_end)
+            assert ens.get(k, "PhysicalTime") == t
+            for a, b, name in zip(ens.state(k), ref, ("dens", "vrad", "vtheta")):
+                assert np.array_equal(a, b), f"disk {k} {name}: {np.abs(a - b).max():.3e}"
+
+
+def test_tiles_do_not_change_results(tmp_path):
+    ds = decks(tmp_path, SMALL[:2])
+    t_end = 2 * 0.314159265359
+    out = []
+    for tiles in ({}, None):
+        with Ensemble(ds, tiles={}) as ens:
+            if tiles is None:
+                ens.set_tiles(0, 8, 64)
+                ens.set_tiles(1, 32, 8)
+            ens.run(t_end)
+            out.append([ens.state(k) for k in range(2)])
+    for k in range(2):
+        for a, b in zip(out[0][k], out[1][k]):
+            assert np.array_equal(a, b)
+
+
+def test_disks_stop_at_the_first_dt_boundary_after_target(tmp_path):
+    ds = decks(tmp_path, SMALL)
+    DT = 0.314159265359
+    with Ensemble(ds, tiles={}) as ens:
+        ens.run(1.5 * DT)
+        for k in range(len(ds)):
+            t = ens.get(k, "PhysicalTime")
+            assert t == pytest.approx(2 * DT, rel=1e-12)
+            assert ens.get(k, "in_dt") == 0.0
+        # a second target continues every disk from there
+        ens.run(3 * DT)
+        for k in range(len(ds)):
+            assert ens.get(k, "PhysicalTime") == pytest.approx(3 * DT, rel=1e-12)
+
+
+def test_drawn_ensemble_runs_and_stays_finite(tmp_path):
+    params = draw(6, seed=5, grids=(16, 24, 32))
+    ds = decks(tmp_path, params)
+    with Ensemble(ds, tiles={}) as ens:
+        ens.run(2 * 0.314159265359)
+        for k in range(len(ds)):
+            dens, vr, vt = ens.state(k)
+            assert np.isfinite(dens).all() and (dens[:-1] > 0).all()
+            assert np.isfinite(vt).all() and np.isfinite(vr).all()
