@@ -30,7 +30,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos", "-Wall",
              f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["fg_context.hip", "fg_steps.hip", "fg_reduce.hip", "fg_sg.hip", "fg_fused.hip"]
+HIP_SOURCES = ["fg_context.hip", "fg_steps.hip", "fg_reduce.hip", "fg_sg.hip", "fg_fused.hip", "fg_mlp.hip"]
 HOST_SOURCES = ["fg_host.cpp", "fg_hostmath.cpp"]
 
 

@@ -130,6 +130,11 @@ class Ensemble:
         call("fg_sim_get_state", self.sim_handle(k), *[a.ctypes.data_as(_pd) for a in out], None)
         return out
 
+    def dens(self, k):
+        out = np.empty(self.shapes[k])
+        call("fg_sim_get_state", self.sim_handle(k), out.ctypes.data_as(_pd), None, None, None)
+        return out
+
     def ring(self, k, name):
         out = np.empty(self.shapes[k][0])
         call("fg_sim_ring", self.sim_handle(k), name.encode(), out.ctypes.data_as(_pd))

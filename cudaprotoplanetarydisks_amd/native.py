@@ -26,6 +26,7 @@ FIELD = {n: k for k, n in enumerate(FIELDS)}
 _d = ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
 _pi = ctypes.POINTER(ctypes.c_int)
+_pf = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -103,6 +104,16 @@ SIGNATURES = {
     "fg_batch_substep": (ctypes.c_int, [_vp, _d, _pi]),
     "fg_batch_run": (ctypes.c_int, [_vp, _d, ctypes.POINTER(ctypes.c_long)]),
     "fg_batch_sim": (_vp, [_vp, ctypes.c_int]),
+    "fg_mlp_create": (ctypes.c_int, [_pi, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int,
+                                     ctypes.POINTER(_vp)]),
+    "fg_mlp_destroy": (ctypes.c_int, [_vp]),
+    "fg_mlp_nparam": (ctypes.c_long, [_vp]),
+    "fg_mlp_get_params": (ctypes.c_int, [_vp, _pf]),
+    "fg_mlp_set_params": (ctypes.c_int, [_vp, _pf]),
+    "fg_mlp_forward": (ctypes.c_int, [_vp, _pf, ctypes.c_int, _pf]),
+    "fg_mlp_gradient": (ctypes.c_int, [_vp, _pf, _pf, ctypes.c_int, _pf, _pf]),
+    "fg_mlp_train": (ctypes.c_int, [_vp, _pf, _pf, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _pf]),
+    "fg_bf16_gemm": (ctypes.c_int, [_pf, _pf, ctypes.c_int, ctypes.c_int, ctypes.c_int, _pf]),
 }
 
 

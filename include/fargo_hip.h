@@ -281,6 +281,31 @@ fg_status fg_batch_run(fg_batch *b, double t_end, long *substeps);
 /* disk k (state, scalars, planets through the fg_sim_* calls) */
 fg_sim *fg_batch_sim(fg_batch *b, int k);
 
+/* ---- learned emulator: MLP on the matrix cores ---------------------- */
+/* Dense layers widths[0] -> widths[1] -> ... -> widths[nlayers], GELU on
+ * the hidden layers, BF16 weights/activations with FP32 accumulation
+ * (v_mfma_f32_16x16x32_bf16), FP32 master weights and Adam moments, MSE
+ * loss.  Parameters are one flat FP32 vector: for each layer W (out x in,
+ * row-major) then b.  Maps disk parameters to ring-averaged profiles for
+ * warm starts; no reference counterpart. */
+typedef struct fg_mlp fg_mlp;
+fg_status fg_mlp_create(const int *widths, int nlayers, int max_batch, unsigned long long seed, int device,
+                        fg_mlp **out);
+fg_status fg_mlp_destroy(fg_mlp *mlp);
+long fg_mlp_nparam(const fg_mlp *mlp);
+fg_status fg_mlp_get_params(fg_mlp *mlp, float *params);
+fg_status fg_mlp_set_params(fg_mlp *mlp, const float *params);
+/* y (batch x outputs) = MLP(x (batch x inputs)), row-major FP32 */
+fg_status fg_mlp_forward(fg_mlp *mlp, const float *x, int batch, float *y);
+/* loss = mean((y - t)^2) and its gradient (flat, parameter order) */
+fg_status fg_mlp_gradient(fg_mlp *mlp, const float *x, const float *t, int batch, float *grad, float *loss);
+/* `steps` Adam steps (lr, beta 0.9/0.999) over n samples in batches of
+ * `batch` rows; losses[s] = loss of step s (may be NULL) */
+fg_status fg_mlp_train(fg_mlp *mlp, const float *x, const float *t, int n, int batch, int steps, float lr,
+                       float *losses);
+/* C = A B with A, B rounded to BF16, FP32 accumulation (the layer GEMM) */
+fg_status fg_bf16_gemm(const float *A, const float *B, int M, int N, int K, float *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "fargo_hip.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:fg_status|const char \*|int|fg_ctx \*)\s*(fg_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:fg_status|const char \*|int|long|fg_ctx \*|fg_sim \*)\s*(fg_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_the_substep_path():
