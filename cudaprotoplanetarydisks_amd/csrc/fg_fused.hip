@@ -108,16 +108,38 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
 #define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
 
   // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
-  for (int q = tid; q < HR * HC; q += SRC_TPB) {
+  // all of a thread's loads are issued before any of them is used
+  double ls[SRC_PER_THREAD], lvr[SRC_PER_THREAD], lvt[SRC_PER_THREAD];
+  int lj[SRC_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < SRC_PER_THREAD; k++) {
+    const int q = tid + k * SRC_TPB;
+    ls[k] = lvr[k] = lvt[k] = 0.0;
+    lj[k] = 0;
+    if (q < HR * HC) {
+      const int r = q / HC, cc = q - r * HC, i = GI(r);
+      if (i >= 0 && i <= nr) {
+        const int j = GJ(cc);
+        const long l = (long)i * ns + j;
+        lj[k] = j;
+        ls[k] = a.dens[l];
+        lvr[k] = a.vr[l];
+        lvt[k] = a.vt[l];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < SRC_PER_THREAD; k++) {
+    const int q = tid + k * SRC_TPB;
+    if (q >= HR * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     double s = 0.0, vr = 0.0, vt = 0.0, p = 0.0, phi = 0.0;
     if (i >= 0 && i <= nr) {
-      int j = GJ(cc);
-      long l = (long)i * ns + j;
-      s = a.dens[l];
-      vr = a.vr[l];
-      vt = a.vt[l];
+      const int j = lj[k];
+      s = ls[k];
+      vr = lvr[k];
+      vt = lvt[k];
       if (i < nr) {
         if (a.corotating) vt = vt - a.domega * t.Rmed[i];
         if (a.evanescent && t.ev_zone[i]) {
@@ -339,7 +361,26 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
   double sg[3] = {}, dqs[2] = {}, vr[3] = {};
   int m0 = max(b0 - 2, 0);
+  // ring m's inputs are loaded one iteration ahead
+  double n_s = 0.0, n_v0 = 0.0, n_v1 = 0.0, n_vtp = 0.0, n_vt = 0.0;
+  if (m0 < nr) {
+    const long l = (long)m0 * ns + j;
+    n_s = a.dens[l];
+    n_v0 = a.vr[l];
+    n_v1 = a.vr[l + ns];
+    n_vtp = a.vt[(long)m0 * ns + jp];
+    n_vt = a.vt[l];
+  }
   for (int m = m0; m <= b1 + 1; m++) {
+    const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
+    if (m + 1 < nr) {
+      const long l = (long)(m + 1) * ns + j;
+      n_s = a.dens[l];
+      n_v0 = a.vr[l];
+      n_v1 = a.vr[l + ns];
+      n_vtp = a.vt[(long)(m + 1) * ns + jp];
+      n_vt = a.vt[l];
+    }
     // shift the window
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
@@ -352,14 +393,13 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     vr[0] = vr[1];
     vr[1] = vr[2];
     if (m < nr) {
-      const long l = (long)m * ns + j;
-      const double s = a.dens[l];
-      const double v0 = a.vr[l], v1 = a.vr[l + ns];
+      const double s = c_s;
+      const double v0 = c_v0, v1 = c_v1;
       const double Rm = t.Rmed[m];
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
-      Q[2][2] = s * (a.vt[(long)m * ns + jp] + Rm * OF) * Rm;
-      Q[3][2] = s * (a.vt[l] + Rm * OF) * Rm;
+      Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
+      Q[3][2] = s * (c_vt + Rm * OF) * Rm;
       Q[4][2] = s;
       sg[2] = s;
       vr[2] = v0;
@@ -562,9 +602,16 @@ __device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
   return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
 }
 
-// one ring; ring holds NSEC doubles (the ring's final vtheta)
+#define FIN_TPB 512
+
+// one ring; ring holds NSEC doubles (the ring's final vtheta).  Thread t
+// owns sectors t + FIN_TPB k (k < C): every load of the ring and of the next
+// ring's radial momenta is issued before use, and the CFL terms that do
+// not involve the ring mean are formed while lane 0 adds the ring in
+// sector order.
+template <int C>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
-  __shared__ double red[256];
+  __shared__ double red[FIN_TPB];
   __shared__ double s_vmoy;
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
@@ -574,19 +621,48 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
   if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i);
   if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
-  for (int j = tid; j < ns; j += 256) {
-    long l = base + j;
-    double vr = recover_vr(a, i, j), vt = a.xvt[l], s = a.xd[l];
+  const bool has_in = i > 0 && i < nr, has_next = i + 1 < nr;
+  double pm[C], mm[C], d0[C], dm[C], xv[C], pr[C], pn[C], mn[C], dn[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    const int j = tid + FIN_TPB * k;
+    pm[k] = mm[k] = d0[k] = dm[k] = xv[k] = pr[k] = pn[k] = mn[k] = dn[k] = 0.0;
+    if (j < ns) {
+      const long l = base + j;
+      d0[k] = a.xd[l];
+      xv[k] = a.xvt[l];
+      if (a.temp) pr[k] = a.press[l];
+      if (has_in) {
+        pm[k] = a.xrmp[l - ns];
+        mm[k] = a.xrmm[l];
+        dm[k] = a.xd[l - ns];
+      }
+      if (has_next) {
+        pn[k] = a.xrmp[l];
+        mn[k] = a.xrmm[l + ns];
+        dn[k] = a.xd[l + ns];
+      }
+    }
+  }
+  double vrk[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    const int j = tid + FIN_TPB * k;
+    if (j >= ns) continue;
+    const long l = base + j;
+    // vrad recovery, Kernels.cu:1200-1205
+    double vr = has_in ? (pm[k] + mm[k]) / (d0[k] + dm[k] + 1e-20) : 0.0, vt = xv[k], sg = d0[k];
     if (damp_i) {
       vr = damp_toward(vr, lam_i, vr0_i);
       vt = damp_toward(vt, lam_i, vt0_i);
-      s = damp_toward(s, lam_i, t.SigmaMed[i]);
+      sg = damp_toward(sg, lam_i, t.SigmaMed[i]);
     }
-    a.dens[l] = s;
+    vrk[k] = vr;
+    a.dens[l] = sg;
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
-    if (a.temp) a.temp[l] = a.press[l] / s;
+    if (a.temp) a.temp[l] = pr[k] / sg;
     if (i == nr - 1) {  // ghost ring after transport: all zero
       a.dens[l + ns] = 0.0;
       a.vr[l + ns] = 0.0;
@@ -601,45 +677,60 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     a.vmoy[i] = m;
     s_vmoy = m;
   }
-  __syncthreads();
-  if (i == 0) return;
-  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
+  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329); the sum
+  // under the square root keeps its order ((dt1^2 + dt2^2) + dt3^2) + dt4^2
   const double dxrad = t.Rsup[i] - t.Rinf[i];
   const double dxtheta = t.dxtheta_cfl[i];
   const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
-  const double vm = s_vmoy, cs = t.cs_iso[i];
+  const double cs = t.cs_iso[i];
+  double p12[C], d4[C];
+  if (i > 0) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      const int j = tid + FIN_TPB * k;
+      if (j >= ns) continue;
+      double vrn = has_next ? (pn[k] + mn[k]) / (dn[k] + d0[k] + 1e-20) : 0.0;
+      if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+      const double vr = vrk[k];
+      double invdt1 = cs / mindx;
+      double invdt2 = fabs(vr) / dxrad;
+      double dvr = vrn - vr;
+      double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
+      dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
+      dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
+      double x = dvr / dxrad, y = dvt / dxtheta;
+      double invdt4 = (y > x) ? y : x;
+      invdt4 *= 4.0 * 1.41 * 1.41;
+      p12[k] = invdt1 * invdt1 + invdt2 * invdt2;
+      d4[k] = invdt4 * invdt4;
+    }
+  }
+  __syncthreads();
+  if (i == 0) return;
+  const double vm = s_vmoy;
   double best = 1e30;
-  for (int j = tid; j < ns; j += 256) {
-    long l = base + j;
-    double vr = a.vr[l];  // written above by this thread
-    double vrn = recover_vr(a, i + 1, j);
-    if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    const int j = tid + FIN_TPB * k;
+    if (j >= ns) continue;
     double vres = a.fast ? (ring[j] - vm) : ring[j];
-    double invdt1 = cs / mindx;
-    double invdt2 = fabs(vr) / dxrad;
     double invdt3 = fabs(vres) / dxtheta;
-    double dvr = vrn - vr;
-    double dvt = ring[(j + 1) % ns] - ring[j];
-    dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
-    dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
-    double x = dvr / dxrad, y = dvt / dxtheta;
-    double invdt4 = (y > x) ? y : x;
-    invdt4 *= 4.0 * 1.41 * 1.41;
-    double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
+    double dtc = 0.5 / sqrt(p12[k] + invdt3 * invdt3 + d4[k]);
     if (dtc < best) best = dtc;
   }
   red[tid] = best;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = FIN_TPB / 2; s > 0; s >>= 1) {
     if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
     __syncthreads();
   }
   if (tid == 0) a.newdt[i] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_fin(fin_args a) {
+template <int C>
+__global__ __launch_bounds__(FIN_TPB) void k_fin(fin_args a) {
   extern __shared__ double lds_fin[];
-  fin_ring(a, blockIdx.x, lds_fin);
+  fin_ring<C>(a, blockIdx.x, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -694,13 +785,9 @@ __device__ __forceinline__ float cell_potential_planets_f(const fg_planets_arg &
   return acc;
 }
 
-// state of one cell after CorrectVtheta and the first damping pass (FP64)
-__device__ __forceinline__ void src32_cell_state(const src32_args &a, int i, long l, double &s, double &vr,
-                                                 double &vt) {
+// CorrectVtheta and the first damping pass of one cell's state (FP64)
+__device__ __forceinline__ void src32_adjust(const src32_args &a, int i, double &s, double &vr, double &vt) {
   const fg_dev_tables &t = a.t;
-  s = a.dens[l];
-  vr = a.vr[l];
-  vt = a.vt[l];
   if (i < a.nr) {
     if (a.corotating) vt = vt - a.domega * t.Rmed[i];
     if (a.evanescent && t.ev_zone[i]) {
@@ -712,6 +799,8 @@ __device__ __forceinline__ void src32_cell_state(const src32_args &a, int i, lon
     }
   }
 }
+
+#define SRC32_OWN_MAX (SRC32_MAX_TH * TW / SRC_TPB)  // own cells per thread
 
 __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, int th, float *lds) {
   const int hr = th + HALO_R + HALO_B;
@@ -734,14 +823,34 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
 #define VREF(i) (((i) >= 0 && (i) < nr) ? a.vref[i] : 0.0)
 
   // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
-  for (int q = tid; q < hr * HC; q += SRC_TPB) {
+  // all of a thread's loads are issued before any of them is used
+  {
+    double ls[SRC32_PER_THREAD], lvr[SRC32_PER_THREAD], lvt[SRC32_PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < SRC32_PER_THREAD; k++) {
+      const int q = tid + k * SRC_TPB;
+      ls[k] = lvr[k] = lvt[k] = 0.0;
+      if (q < hr * HC) {
+        const int r = q / HC, cc = q - r * HC, i = GI(r);
+        if (i >= 0 && i <= nr) {
+          const long l = (long)i * ns + GJ(cc);
+          ls[k] = a.dens[l];
+          lvr[k] = a.vr[l];
+          lvt[k] = a.vt[l];
+        }
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < SRC32_PER_THREAD; k++) {
+    const int q = tid + k * SRC_TPB;
+    if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
     if (i >= 0 && i <= nr) {
       int j = GJ(cc);
-      double sd, vrd, vtd;
-      src32_cell_state(a, i, (long)i * ns + j, sd, vrd, vtd);
+      double sd = ls[k], vrd = lvr[k], vtd = lvt[k];
+      src32_adjust(a, i, sd, vrd, vtd);
       s = (float)sd;
       vr = (float)vrd;
       vt = (float)(vtd - VREF(i));
@@ -756,6 +865,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     sB[r][cc] = phi;
     sC[r][cc] = vt;
     sD[r][cc] = vr;
+  }
   }
   __syncthreads();
 
@@ -873,14 +983,32 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   __syncthreads();
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
-  for (int q = tid; q < th * TW; q += SRC_TPB) {
+  double os[SRC32_OWN_MAX], ovr[SRC32_OWN_MAX], ovt[SRC32_OWN_MAX];
+#pragma unroll
+  for (int k = 0; k < SRC32_OWN_MAX; k++) {
+    const int q = tid + k * SRC_TPB;
+    os[k] = ovr[k] = ovt[k] = 0.0;
+    if (q < th * TW) {
+      const int i = GI(q / TW + HALO_R), j = j0 + q % TW;
+      if (i <= nr && j < ns) {
+        const long l = (long)i * ns + j;
+        os[k] = a.dens[l];
+        ovr[k] = a.vr[l];
+        ovt[k] = a.vt[l];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < SRC32_OWN_MAX; k++) {
+    const int q = tid + k * SRC_TPB;
+    if (q >= th * TW) break;
     int ro = q / TW, co = q % TW;
     int r = ro + HALO_R, cc = co + HALO_C;
     int i = GI(r), j = j0 + co;
     if (i > nr || j >= ns) continue;
     long l = (long)i * ns + j;
-    double s, vr, vt;
-    src32_cell_state(a, i, l, s, vr, vt);
+    double s = os[k], vr = ovr[k], vt = ovt[k];
+    src32_adjust(a, i, s, vr, vt);
     if (a.opress && i < nr) a.opress[l] = s * t.cs_iso[i] * t.cs_iso[i];
     if (i == nr) {  // ghost ring
       a.odens[l] = s;
@@ -941,7 +1069,26 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   float W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
   float sg[3] = {}, dqs[2] = {}, vr[3] = {};
   int m0 = max(b0 - 2, 0);
+  // ring m's inputs are loaded one iteration ahead
+  double n_s = 0.0, n_v0 = 0.0, n_v1 = 0.0, n_vtp = 0.0, n_vt = 0.0;
+  if (m0 < nr) {
+    const long l = (long)m0 * ns + j;
+    n_s = a.dens[l];
+    n_v0 = a.vr[l];
+    n_v1 = a.vr[l + ns];
+    n_vtp = a.vt[(long)m0 * ns + jp];
+    n_vt = a.vt[l];
+  }
   for (int m = m0; m <= b1 + 1; m++) {
+    const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
+    if (m + 1 < nr) {
+      const long l = (long)(m + 1) * ns + j;
+      n_s = a.dens[l];
+      n_v0 = a.vr[l];
+      n_v1 = a.vr[l + ns];
+      n_vtp = a.vt[(long)(m + 1) * ns + jp];
+      n_vt = a.vt[l];
+    }
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
       Q[q][1] = Q[q][2];
@@ -953,14 +1100,13 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     vr[0] = vr[1];
     vr[1] = vr[2];
     if (m < nr) {
-      const long l = (long)m * ns + j;
-      const double s = a.dens[l];
-      const double v0 = a.vr[l], v1 = a.vr[l + ns];
+      const double s = c_s;
+      const double v0 = c_v0, v1 = c_v1;
       const double Rm = t.Rmed[m];
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
-      Q[2][2] = s * (a.vt[(long)m * ns + jp] + Rm * OF) * Rm;
-      Q[3][2] = s * (a.vt[l] + Rm * OF) * Rm;
+      Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
+      Q[3][2] = s * (c_vt + Rm * OF) * Rm;
       Q[4][2] = s;
       const float fs = (float)s;
       const float rs = rcp_f(fs + 1e-20f);
@@ -1258,13 +1404,16 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   return FG_OK;
 }
 
-// cells per thread of k_taz: the smallest instantiated C with 256 C >= NSEC
-static int taz_cells(int ns) {
+// cells per thread of k_taz / k_fin: the smallest instantiated C with
+// (threads per ring) * C >= NSEC
+static int ring_cells(int ns, int tpb) {
   const int cs[] = {1, 2, 3, 4, 6, 8, 12, 16};
   for (int c : cs)
-    if (TAZ_TPB * c >= ns) return c;
+    if (tpb * c >= ns) return c;
   return 0;
 }
+static int taz_cells(int ns) { return ring_cells(ns, TAZ_TPB); }
+static int fin_cells(int ns) { return ring_cells(ns, FIN_TPB); }
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
@@ -1433,7 +1582,20 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
   prof_mark(c);
-  hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
+  {
+    const size_t shf = (size_t)ns * sizeof(double);
+    const dim3 gf(nr), bf(FIN_TPB);
+    switch (fin_cells(ns)) {
+      case 1: hipLaunchKernelGGL(k_fin<1>, gf, bf, shf, c->stream, fa); break;
+      case 2: hipLaunchKernelGGL(k_fin<2>, gf, bf, shf, c->stream, fa); break;
+      case 3: hipLaunchKernelGGL(k_fin<3>, gf, bf, shf, c->stream, fa); break;
+      case 4: hipLaunchKernelGGL(k_fin<4>, gf, bf, shf, c->stream, fa); break;
+      case 6: hipLaunchKernelGGL(k_fin<6>, gf, bf, shf, c->stream, fa); break;
+      case 8: hipLaunchKernelGGL(k_fin<8>, gf, bf, shf, c->stream, fa); break;
+      case 12: hipLaunchKernelGGL(k_fin<12>, gf, bf, shf, c->stream, fa); break;
+      default: hipLaunchKernelGGL(k_fin<16>, gf, bf, shf, c->stream, fa); break;
+    }
+  }
   FG_LAUNCH_CHECK();
   prof_mark(c);
   return fused_reductions(c, DeltaT_next);
@@ -1473,45 +1635,56 @@ struct batch_result {
 
 #define BATCH_CAP_BLOCKS 4096  // workgroups of a persistent phase launch
 
-__global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *D, const int4 *items, int n) {
+// work items are the same for the whole workgroup: keep them (and every
+// address derived from them) in scalar registers
+__device__ __forceinline__ int4 uniform4(int4 w) {
+  return make_int4(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y),
+                   __builtin_amdgcn_readfirstlane(w.z), __builtin_amdgcn_readfirstlane(w.w));
+}
+__device__ __forceinline__ int2 uniform2(int2 w) {
+  return make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y));
+}
+
+__global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
   extern __shared__ float lds_bsrc[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int4 w = items[it];
+    const int4 w = uniform4(items[it]);
     if (D[w.x].active) src32_tile(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *D, const int2 *items, int n) {
+__global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
+    const int2 w = uniform2(items[it]);
     if (D[w.x].active) trad32_block(D[w.x].trad, w.y);
     __syncthreads();
   }
 }
 
 template <int C>
-__global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *D, const int2 *items, int n) {
+__global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   extern __shared__ double lds_btaz[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
+    const int2 w = uniform2(items[it]);
     if (D[w.x].active) taz32_ring<C>(D[w.x].taz, w.y, lds_btaz);
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_fin(const batch_disk *D, const int2 *items, int n) {
+template <int C>
+__global__ __launch_bounds__(FIN_TPB) void k_batch_fin(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
-    if (D[w.x].active) fin_ring(D[w.x].fin, w.y, lds_bfin);
+    const int2 w = uniform2(items[it]);
+    if (D[w.x].active) fin_ring<C>(D[w.x].fin, w.y, lds_bfin);
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(RED_TPB) void k_batch_force(const batch_disk *D, const int2 *items, int n) {
+__global__ __launch_bounds__(RED_TPB) void k_batch_force(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
+    const int2 w = uniform2(items[it]);
     const batch_disk &d = D[w.x];
     if (d.active) dreduce_block(d.force, d.ncell, d.B, w.y, d.partial);
     __syncthreads();
@@ -1519,7 +1692,7 @@ __global__ __launch_bounds__(RED_TPB) void k_batch_force(const batch_disk *D, co
 }
 
 // per disk: the force's final sum and the CFL count of the next substep
-__global__ __launch_bounds__(256) void k_batch_close(const batch_disk *D, batch_result *res, int ndisk) {
+__global__ __launch_bounds__(256) void k_batch_close(const batch_disk *__restrict__ D, batch_result *__restrict__ res, int ndisk) {
   __shared__ double out[2];
   for (int k = blockIdx.x; k < ndisk; k += gridDim.x) {
     const batch_disk &d = D[k];
@@ -1683,8 +1856,20 @@ static fg_status batch_build(fg_batch_dev *b) {
     case 12: hipLaunchKernelGGL(k_batch_taz<12>, gt, bt, shm_taz, st, D, IT, NT); break;
     default: hipLaunchKernelGGL(k_batch_taz<16>, gt, bt, shm_taz, st, D, IT, NT); break;
   }
-  hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
-                     b->it_fin, b->n_fin);
+  const size_t shm_fin = (size_t)b->max_ns * sizeof(double);
+  const dim3 gfin = grid(b->n_fin), bfin(FIN_TPB);
+  const int2 *IF = b->it_fin;
+  const int NF = b->n_fin;
+  switch (fin_cells(b->max_ns)) {
+    case 1: hipLaunchKernelGGL(k_batch_fin<1>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 2: hipLaunchKernelGGL(k_batch_fin<2>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 3: hipLaunchKernelGGL(k_batch_fin<3>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 4: hipLaunchKernelGGL(k_batch_fin<4>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 6: hipLaunchKernelGGL(k_batch_fin<6>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 8: hipLaunchKernelGGL(k_batch_fin<8>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    case 12: hipLaunchKernelGGL(k_batch_fin<12>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+    default: hipLaunchKernelGGL(k_batch_fin<16>, gfin, bfin, shm_fin, st, D, IF, NF); break;
+  }
   hipLaunchKernelGGL(k_batch_force, grid(b->n_force), dim3(RED_TPB), 0, st, b->d_disk, b->it_force, b->n_force);
   hipLaunchKernelGGL(k_batch_close, grid(b->n), dim3(256), 0, st, b->d_disk, b->d_res, b->n);
   (void)hipMemcpyAsync(b->h_res, b->d_res, b->n * sizeof(batch_result), hipMemcpyDeviceToHost, st);
