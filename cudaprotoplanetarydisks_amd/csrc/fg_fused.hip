@@ -602,16 +602,9 @@ __device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
   return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
 }
 
-#define FIN_TPB 512
-
-// one ring; ring holds NSEC doubles (the ring's final vtheta).  Thread t
-// owns sectors t + FIN_TPB k (k < C): every load of the ring and of the next
-// ring's radial momenta is issued before use, and the CFL terms that do
-// not involve the ring mean are formed while lane 0 adds the ring in
-// sector order.
-template <int C>
+// one ring; ring holds NSEC doubles (the ring's final vtheta)
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
-  __shared__ double red[FIN_TPB];
+  __shared__ double red[256];
   __shared__ double s_vmoy;
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
@@ -621,48 +614,19 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
   if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i);
   if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
-  const bool has_in = i > 0 && i < nr, has_next = i + 1 < nr;
-  double pm[C], mm[C], d0[C], dm[C], xv[C], pr[C], pn[C], mn[C], dn[C];
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    const int j = tid + FIN_TPB * k;
-    pm[k] = mm[k] = d0[k] = dm[k] = xv[k] = pr[k] = pn[k] = mn[k] = dn[k] = 0.0;
-    if (j < ns) {
-      const long l = base + j;
-      d0[k] = a.xd[l];
-      xv[k] = a.xvt[l];
-      if (a.temp) pr[k] = a.press[l];
-      if (has_in) {
-        pm[k] = a.xrmp[l - ns];
-        mm[k] = a.xrmm[l];
-        dm[k] = a.xd[l - ns];
-      }
-      if (has_next) {
-        pn[k] = a.xrmp[l];
-        mn[k] = a.xrmm[l + ns];
-        dn[k] = a.xd[l + ns];
-      }
-    }
-  }
-  double vrk[C];
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    const int j = tid + FIN_TPB * k;
-    if (j >= ns) continue;
-    const long l = base + j;
-    // vrad recovery, Kernels.cu:1200-1205
-    double vr = has_in ? (pm[k] + mm[k]) / (d0[k] + dm[k] + 1e-20) : 0.0, vt = xv[k], sg = d0[k];
+  for (int j = tid; j < ns; j += 256) {
+    long l = base + j;
+    double vr = recover_vr(a, i, j), vt = a.xvt[l], s = a.xd[l];
     if (damp_i) {
       vr = damp_toward(vr, lam_i, vr0_i);
       vt = damp_toward(vt, lam_i, vt0_i);
-      sg = damp_toward(sg, lam_i, t.SigmaMed[i]);
+      s = damp_toward(s, lam_i, t.SigmaMed[i]);
     }
-    vrk[k] = vr;
-    a.dens[l] = sg;
+    a.dens[l] = s;
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
-    if (a.temp) a.temp[l] = pr[k] / sg;
+    if (a.temp) a.temp[l] = a.press[l] / s;
     if (i == nr - 1) {  // ghost ring after transport: all zero
       a.dens[l + ns] = 0.0;
       a.vr[l + ns] = 0.0;
@@ -677,60 +641,45 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     a.vmoy[i] = m;
     s_vmoy = m;
   }
-  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329); the sum
-  // under the square root keeps its order ((dt1^2 + dt2^2) + dt3^2) + dt4^2
+  __syncthreads();
+  if (i == 0) return;
+  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
   const double dxrad = t.Rsup[i] - t.Rinf[i];
   const double dxtheta = t.dxtheta_cfl[i];
   const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
-  const double cs = t.cs_iso[i];
-  double p12[C], d4[C];
-  if (i > 0) {
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-      const int j = tid + FIN_TPB * k;
-      if (j >= ns) continue;
-      double vrn = has_next ? (pn[k] + mn[k]) / (dn[k] + d0[k] + 1e-20) : 0.0;
-      if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
-      const double vr = vrk[k];
-      double invdt1 = cs / mindx;
-      double invdt2 = fabs(vr) / dxrad;
-      double dvr = vrn - vr;
-      double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
-      dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
-      dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
-      double x = dvr / dxrad, y = dvt / dxtheta;
-      double invdt4 = (y > x) ? y : x;
-      invdt4 *= 4.0 * 1.41 * 1.41;
-      p12[k] = invdt1 * invdt1 + invdt2 * invdt2;
-      d4[k] = invdt4 * invdt4;
-    }
-  }
-  __syncthreads();
-  if (i == 0) return;
-  const double vm = s_vmoy;
+  const double vm = s_vmoy, cs = t.cs_iso[i];
   double best = 1e30;
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    const int j = tid + FIN_TPB * k;
-    if (j >= ns) continue;
+  for (int j = tid; j < ns; j += 256) {
+    long l = base + j;
+    double vr = a.vr[l];  // written above by this thread
+    double vrn = recover_vr(a, i + 1, j);
+    if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
     double vres = a.fast ? (ring[j] - vm) : ring[j];
+    double invdt1 = cs / mindx;
+    double invdt2 = fabs(vr) / dxrad;
     double invdt3 = fabs(vres) / dxtheta;
-    double dtc = 0.5 / sqrt(p12[k] + invdt3 * invdt3 + d4[k]);
+    double dvr = vrn - vr;
+    double dvt = ring[(j + 1) % ns] - ring[j];
+    dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
+    dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
+    double x = dvr / dxrad, y = dvt / dxtheta;
+    double invdt4 = (y > x) ? y : x;
+    invdt4 *= 4.0 * 1.41 * 1.41;
+    double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
     if (dtc < best) best = dtc;
   }
   red[tid] = best;
   __syncthreads();
-  for (int s = FIN_TPB / 2; s > 0; s >>= 1) {
+  for (int s = 128; s > 0; s >>= 1) {
     if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
     __syncthreads();
   }
   if (tid == 0) a.newdt[i] = red[0];
 }
 
-template <int C>
-__global__ __launch_bounds__(FIN_TPB) void k_fin(fin_args a) {
+__global__ __launch_bounds__(256) void k_fin(fin_args a) {
   extern __shared__ double lds_fin[];
-  fin_ring<C>(a, blockIdx.x, lds_fin);
+  fin_ring(a, blockIdx.x, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -1404,7 +1353,7 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   return FG_OK;
 }
 
-// cells per thread of k_taz / k_fin: the smallest instantiated C with
+// cells per thread of k_taz: the smallest instantiated C with
 // (threads per ring) * C >= NSEC
 static int ring_cells(int ns, int tpb) {
   const int cs[] = {1, 2, 3, 4, 6, 8, 12, 16};
@@ -1413,7 +1362,6 @@ static int ring_cells(int ns, int tpb) {
   return 0;
 }
 static int taz_cells(int ns) { return ring_cells(ns, TAZ_TPB); }
-static int fin_cells(int ns) { return ring_cells(ns, FIN_TPB); }
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
@@ -1582,20 +1530,7 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
   prof_mark(c);
-  {
-    const size_t shf = (size_t)ns * sizeof(double);
-    const dim3 gf(nr), bf(FIN_TPB);
-    switch (fin_cells(ns)) {
-      case 1: hipLaunchKernelGGL(k_fin<1>, gf, bf, shf, c->stream, fa); break;
-      case 2: hipLaunchKernelGGL(k_fin<2>, gf, bf, shf, c->stream, fa); break;
-      case 3: hipLaunchKernelGGL(k_fin<3>, gf, bf, shf, c->stream, fa); break;
-      case 4: hipLaunchKernelGGL(k_fin<4>, gf, bf, shf, c->stream, fa); break;
-      case 6: hipLaunchKernelGGL(k_fin<6>, gf, bf, shf, c->stream, fa); break;
-      case 8: hipLaunchKernelGGL(k_fin<8>, gf, bf, shf, c->stream, fa); break;
-      case 12: hipLaunchKernelGGL(k_fin<12>, gf, bf, shf, c->stream, fa); break;
-      default: hipLaunchKernelGGL(k_fin<16>, gf, bf, shf, c->stream, fa); break;
-    }
-  }
+  hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   FG_LAUNCH_CHECK();
   prof_mark(c);
   return fused_reductions(c, DeltaT_next);
@@ -1672,12 +1607,11 @@ __global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *__restr
   }
 }
 
-template <int C>
-__global__ __launch_bounds__(FIN_TPB) void k_batch_fin(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
+__global__ __launch_bounds__(256) void k_batch_fin(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) fin_ring<C>(D[w.x].fin, w.y, lds_bfin);
+    if (D[w.x].active) fin_ring(D[w.x].fin, w.y, lds_bfin);
     __syncthreads();
   }
 }
@@ -1856,20 +1790,8 @@ static fg_status batch_build(fg_batch_dev *b) {
     case 12: hipLaunchKernelGGL(k_batch_taz<12>, gt, bt, shm_taz, st, D, IT, NT); break;
     default: hipLaunchKernelGGL(k_batch_taz<16>, gt, bt, shm_taz, st, D, IT, NT); break;
   }
-  const size_t shm_fin = (size_t)b->max_ns * sizeof(double);
-  const dim3 gfin = grid(b->n_fin), bfin(FIN_TPB);
-  const int2 *IF = b->it_fin;
-  const int NF = b->n_fin;
-  switch (fin_cells(b->max_ns)) {
-    case 1: hipLaunchKernelGGL(k_batch_fin<1>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 2: hipLaunchKernelGGL(k_batch_fin<2>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 3: hipLaunchKernelGGL(k_batch_fin<3>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 4: hipLaunchKernelGGL(k_batch_fin<4>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 6: hipLaunchKernelGGL(k_batch_fin<6>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 8: hipLaunchKernelGGL(k_batch_fin<8>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    case 12: hipLaunchKernelGGL(k_batch_fin<12>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-    default: hipLaunchKernelGGL(k_batch_fin<16>, gfin, bfin, shm_fin, st, D, IF, NF); break;
-  }
+  hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
+                     b->it_fin, b->n_fin);
   hipLaunchKernelGGL(k_batch_force, grid(b->n_force), dim3(RED_TPB), 0, st, b->d_disk, b->it_force, b->n_force);
   hipLaunchKernelGGL(k_batch_close, grid(b->n), dim3(256), 0, st, b->d_disk, b->d_res, b->n);
   (void)hipMemcpyAsync(b->h_res, b->d_res, b->n * sizeof(batch_result), hipMemcpyDeviceToHost, st);
