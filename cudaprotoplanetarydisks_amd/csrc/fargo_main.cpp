@@ -16,6 +16,7 @@ static void usage(const char *exe) {  // PrintUsage, Interpret.cu:216-239
   printf("\n-c : Sloppy CFL condition (checked at each DT, not at each timestep)\n");
   printf("-n : Disable simulation. The program just reads parameters file\n");
   printf("-o : Overrides output directory of input file.\n");
+  printf("-s N : Restart from output set N of the output directory\n");
   printf("-F : (this build) use the fused substep kernels\n");
   printf("-S : (this build) FP32 bulk precision mode\n");
   exit(1);
@@ -33,7 +34,7 @@ int main(int argc, char **argv) {
   }
   if (argc == 1) usage(argv[0]);
   std::string flags, outdir, parfile;
-  int disable = 0, fused = 0, fp32 = 0;
+  int disable = 0, fused = 0, fp32 = 0, restart = -1;
   for (int i = 1; i < argc; i++) {
     if (argv[i][0] == '-') {
       if (strspn(argv[i], "-secndovtpfamzib0123456789FS") != strlen(argv[i])) usage(argv[0]);
@@ -41,11 +42,11 @@ int main(int argc, char **argv) {
       if (strchr(argv[i], 'n')) disable = 1;
       if (strchr(argv[i], 'F')) fused = 1;
       if (strchr(argv[i], 'S')) fp32 = 1;
-      if (strchr(argv[i], 's')) {
-        fprintf(stderr, "-s (restart) is not implemented by the reference either (Main.cu:152)\n");
-        return 1;
-      }
-      if (strchr(argv[i], 'o')) {
+      if (strchr(argv[i], 's')) {  // Main.cu:152-160
+        if (++i >= argc) usage(argv[0]);
+        restart = atoi(argv[i]);
+        if (restart < 0) usage(argv[0]);
+      } else if (strchr(argv[i], 'o')) {
         if (++i >= argc) usage(argv[0]);
         outdir = argv[i];
       } else if (strchr(argv[i], 'f')) {
@@ -76,7 +77,19 @@ int main(int argc, char **argv) {
   fg_sim_get(sim, "NTOT", &ntot);
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
-  st = fg_sim_run(sim, 0, (int)ntot);
+  double ninterm = 1;
+  fg_sim_get(sim, "NINTERM", &ninterm);
+  int it0 = 0;
+  if (restart >= 0) {
+    st = fg_sim_restart(sim, restart);
+    if (st) {
+      fprintf(stderr, "fargo_mi355x: %s\n", fg_last_error());
+      fg_sim_destroy(sim);
+      return 1;
+    }
+    it0 = restart * (int)ninterm;
+  }
+  st = fg_sim_run(sim, it0, (int)ntot);
   clock_gettime(CLOCK_MONOTONIC, &t1);
   if (st) {
     fprintf(stderr, "fargo_mi355x: %s\n", fg_last_error());

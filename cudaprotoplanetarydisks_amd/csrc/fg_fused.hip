@@ -40,6 +40,7 @@
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
 
+
 __global__ void k_cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt, const double *invRmed,
                             double coef, double DeltaT, int nr);
 
@@ -467,14 +468,19 @@ struct taz_args {
   int nr, ns, fast;
 };
 
+// cells of thread t: j = t + TAZ_TPB k, k < C (all of them when FULL,
+// i.e. NSEC == TAZ_TPB * C)
 #define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
 
 // QuantitiesAdvection (TransportEuler.cu:229-245) with velocity V (one
-// value per cell, or the ring's uniform residual when UNIFORM)
-template <int C, bool UNIFORM>
+// value per cell, or the ring's uniform residual when UNIFORM).  W, D and
+// F are three NSEC-long LDS rings (cell values, slopes, fluxes): with a
+// separate flux ring each quantity needs three barriers.
+template <int C, bool FULL, bool UNIFORM>
 __device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double (&Vc)[C], double Vu, double *W,
-                                                double *D, int ns, int i, const fg_dev_tables &t, double dt) {
+                                                double *D, double *Fl, int ns, int i, const fg_dev_tables &t,
+                                                double dt) {
   const double invdxtheta = t.invdxtheta[i], dxtheta = t.dxtheta[i];
   const double dxrad = (t.Rsup[i] - t.Rinf[i]) * dt;
   const double invsurf = t.invSurf[i];
@@ -505,24 +511,24 @@ __device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double
       int jm = j == 0 ? ns - 1 : j - 1;
       double QS = vl_theta_star(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta);
       F[k] = dxrad * QS * DS[k] * V(k);
+      Fl[j] = F[k];
     }
-    __syncthreads();
-    FOR_CELLS(k, j) W[j] = F[k];
     __syncthreads();
     FOR_CELLS(k, j) {
       double varq = F[k];
-      varq -= W[j + 1 == ns ? 0 : j + 1];
+      varq -= Fl[j + 1 == ns ? 0 : j + 1];
       q[qq][k] += varq * invsurf;
     }
-    __syncthreads();
   }
+  __syncthreads();
 #undef V
 }
 
-template <int C>
+// lds holds three NSEC-long double rings
+template <int C, bool FULL>
 __global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
   extern __shared__ double lds[];
-  double *W = lds, *D = lds + a.ns;
+  double *W = lds, *D = lds + a.ns, *Fl = lds + 2 * a.ns;
   const int ns = a.ns, i = blockIdx.x, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
@@ -545,22 +551,14 @@ __global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
     vuni = 0.0;
   }
   if (tid == 0) a.nshift[i] = nsh;
-  azimuthal_sweep<C, false>(q, vres, 0.0, W, D, ns, i, t, a.dt);
-  azimuthal_sweep<C, true>(q, vres, vuni, W, D, ns, i, t, a.dt);
-  // AdvectSHIFT (Kernels.cu:1173-1190) of every quantity
+  azimuthal_sweep<C, FULL, false>(q, vres, 0.0, W, D, Fl, ns, i, t, a.dt);
+  azimuthal_sweep<C, FULL, true>(q, vres, vuni, W, D, Fl, ns, i, t, a.dt);
+  // AdvectSHIFT (Kernels.cu:1173-1190) folded into the stores: the value
+  // of cell j goes to j + Nshift (mod NSEC).  vtheta from the angular
+  // momenta (Kernels.cu:1206-1207) of the shifted ring at j + Nshift uses
+  // the cells j - 1 and j of the unshifted one.
   int sh = nsh % ns;
   if (sh < 0) sh += ns;
-#pragma unroll
-  for (int qq = 0; qq < NQ; qq++) {
-    FOR_CELLS(k, j) W[j] = q[qq][k];
-    __syncthreads();
-    FOR_CELLS(k, j) {
-      int ji = j - sh;
-      q[qq][k] = W[ji < 0 ? ji + ns : ji];
-    }
-    __syncthreads();
-  }
-  // vtheta from the angular momenta (Kernels.cu:1206-1207)
   FOR_CELLS(k, j) {
     W[j] = q[2][k];
     D[j] = q[NQ - 1][k];
@@ -571,10 +569,12 @@ __global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
     int jm = j == 0 ? ns - 1 : j - 1;
     double s = q[NQ - 1][k];
     double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
-    a.xd[base + j] = s;
-    a.xvt[base + j] = v;
-    a.xrmp[base + j] = q[0][k];
-    a.xrmm[base + j] = q[1][k];
+    int jo = j + sh;
+    if (jo >= ns) jo -= ns;
+    a.xd[base + jo] = s;
+    a.xvt[base + jo] = v;
+    a.xrmp[base + jo] = q[0][k];
+    a.xrmm[base + jo] = q[1][k];
   }
 }
 #undef FOR_CELLS
@@ -1107,55 +1107,68 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
 
-// Azimuthal sweep in FP32 (azimuthal_sweep above); q stays FP64.
-template <int C, bool UNIFORM>
+// Azimuthal sweep in FP32 (azimuthal_sweep above); q stays FP64.  Each
+// thread keeps its own cells' values and slopes in registers and reads
+// only the neighbours' from the LDS rings W, D, F.
+template <int C, bool FULL, bool UNIFORM>
 __device__ __forceinline__ void azimuthal_sweep32(double (&q)[NQ][C], const float (&Vc)[C], float Vu, float *W,
-                                                  float *D, int ns, int i, const fg_dev_tables &t, float dt) {
+                                                  float *D, float *Fl, int ns, int i, const fg_dev_tables &t,
+                                                  float dt) {
   const float invdxtheta = (float)t.invdxtheta[i], dxtheta = (float)t.dxtheta[i];
   const float dxrad = (float)((t.Rsup[i] - t.Rinf[i]) * (double)dt);
   const float invsurf = (float)t.invSurf[i];
-  float rs[C], DS[C];
+  float rs[C], DS[C], wk[C], dk[C];
 #define V(k) (UNIFORM ? Vu : Vc[k])
 #define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
+#define JM(j) ((j) == 0 ? ns - 1 : (j) - 1)
+#define JP(j) ((j) + 1 == ns ? 0 : (j) + 1)
   FOR_CELLS(k, j) {
     const float d = (float)q[NQ - 1][k];
     rs[k] = rcp_f(d + 1e-20f);
+    wk[k] = d;
     W[j] = d;
   }
   __syncthreads();
-  FOR_CELLS(k, j) D[j] = vl_theta_slope_f(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
-  __syncthreads();
   FOR_CELLS(k, j) {
-    int jm = j == 0 ? ns - 1 : j - 1;
-    DS[k] = vl_theta_star_f(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta) * V(k) * dxrad;
+    dk[k] = vl_theta_slope_f(W[JM(j)], wk[k], W[JP(j)], invdxtheta);
+    D[j] = dk[k];
   }
+  __syncthreads();
+  FOR_CELLS(k, j) DS[k] = vl_theta_star_f(V(k), dt, W[JM(j)], wk[k], D[JM(j)], dk[k], dxtheta) * V(k) * dxrad;
   __syncthreads();
 #pragma unroll
   for (int qq = 0; qq < NQ; qq++) {
-    FOR_CELLS(k, j) W[j] = (float)q[qq][k] * rs[k];
+    FOR_CELLS(k, j) {
+      wk[k] = (float)q[qq][k] * rs[k];
+      W[j] = wk[k];
+    }
     __syncthreads();
-    FOR_CELLS(k, j) D[j] = vl_theta_slope_f(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
+    FOR_CELLS(k, j) {
+      dk[k] = vl_theta_slope_f(W[JM(j)], wk[k], W[JP(j)], invdxtheta);
+      D[j] = dk[k];
+    }
     __syncthreads();
     float F[C];
     FOR_CELLS(k, j) {
-      int jm = j == 0 ? ns - 1 : j - 1;
-      F[k] = vl_theta_star_f(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta) * DS[k];
+      F[k] = vl_theta_star_f(V(k), dt, W[JM(j)], wk[k], D[JM(j)], dk[k], dxtheta) * DS[k];
+      Fl[j] = F[k];
     }
     __syncthreads();
-    FOR_CELLS(k, j) W[j] = F[k];
-    __syncthreads();
-    FOR_CELLS(k, j) q[qq][k] += (double)((F[k] - W[j + 1 == ns ? 0 : j + 1]) * invsurf);
-    __syncthreads();
+    FOR_CELLS(k, j) q[qq][k] += (double)((F[k] - Fl[JP(j)]) * invsurf);
   }
+  __syncthreads();
 #undef V
 #undef FOR_CELLS
+#undef JM
+#undef JP
 }
 
-// one ring; lds holds 2 * NSEC doubles
-template <int C>
+// one ring; lds holds 2 * NSEC doubles (three float rings during the
+// sweeps, two double rings for the vtheta recovery)
+template <int C, bool FULL>
 __device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds) {
-  float *Wf = (float *)lds, *Df = Wf + a.ns;
+  float *Wf = (float *)lds, *Df = Wf + a.ns, *Ff = Df + a.ns;
   double *W = lds, *D = lds + a.ns;
   const int ns = a.ns, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
@@ -1164,7 +1177,7 @@ __device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds
   double q[NQ][C];
   float vres[C];
 #define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (j < ns)
+  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
   // FARGO residual phase in FP64 (Kernels.cu:1084-1126)
   const double invdt = 1.0 / a.dt;
   const double dpinvns = 2.0 * FG_PI / (double)ns;
@@ -1181,20 +1194,11 @@ __device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds
   }
   if (!a.fast) vuni = 0.0;
   if (tid == 0) a.nshift[i] = nsh;
-  azimuthal_sweep32<C, false>(q, vres, 0.0f, Wf, Df, ns, i, t, (float)a.dt);
-  azimuthal_sweep32<C, true>(q, vres, (float)vuni, Wf, Df, ns, i, t, (float)a.dt);
+  azimuthal_sweep32<C, FULL, false>(q, vres, 0.0f, Wf, Df, Ff, ns, i, t, (float)a.dt);
+  azimuthal_sweep32<C, FULL, true>(q, vres, (float)vuni, Wf, Df, Ff, ns, i, t, (float)a.dt);
+  // shift folded into the stores, as in k_taz
   int sh = nsh % ns;
   if (sh < 0) sh += ns;
-#pragma unroll
-  for (int qq = 0; qq < NQ; qq++) {
-    FOR_CELLS(k, j) W[j] = q[qq][k];
-    __syncthreads();
-    FOR_CELLS(k, j) {
-      int ji = j - sh;
-      q[qq][k] = W[ji < 0 ? ji + ns : ji];
-    }
-    __syncthreads();
-  }
   FOR_CELLS(k, j) {
     W[j] = q[2][k];
     D[j] = q[NQ - 1][k];
@@ -1205,18 +1209,20 @@ __device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds
     int jm = j == 0 ? ns - 1 : j - 1;
     double s = q[NQ - 1][k];
     double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
-    a.xd[base + j] = s;
-    a.xvt[base + j] = v;
-    a.xrmp[base + j] = q[0][k];
-    a.xrmm[base + j] = q[1][k];
+    int jo = j + sh;
+    if (jo >= ns) jo -= ns;
+    a.xd[base + jo] = s;
+    a.xvt[base + jo] = v;
+    a.xrmp[base + jo] = q[0][k];
+    a.xrmm[base + jo] = q[1][k];
   }
 #undef FOR_CELLS
 }
 
-template <int C>
+template <int C, bool FULL>
 __global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
   extern __shared__ double lds_taz32[];
-  taz32_ring<C>(a, blockIdx.x, lds_taz32);
+  taz32_ring<C, FULL>(a, blockIdx.x, lds_taz32);
 }
 
 // ----------------------------------------------------- entry points ----
@@ -1363,6 +1369,25 @@ static int ring_cells(int ns, int tpb) {
 }
 static int taz_cells(int ns) { return ring_cells(ns, TAZ_TPB); }
 
+// k_taz / k_taz32 with C cells per thread; the FULL variant (no bounds
+// test per cell) when NSEC == TAZ_TPB * C and C <= 6 (for larger C it
+// needs more than 256 VGPRs)
+template <int C>
+static void taz_launch(bool fp32, dim3 g, dim3 blk, size_t shm, hipStream_t st, const taz_args &za) {
+  const bool full = C <= 6 && za.ns == TAZ_TPB * C;
+  if (fp32) {
+    if (full)
+      hipLaunchKernelGGL((k_taz32<C, true>), g, blk, shm, st, za);
+    else
+      hipLaunchKernelGGL((k_taz32<C, false>), g, blk, shm, st, za);
+  } else {
+    if (full)
+      hipLaunchKernelGGL((k_taz<C, true>), g, blk, shm, st, za);
+    else
+      hipLaunchKernelGGL((k_taz<C, false>), g, blk, shm, st, za);
+  }
+}
+
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
                                    double DeltaT_next, int want_temperature) {
@@ -1488,25 +1513,18 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.ns = ns;
   za.fast = c->p.fast_transport;
   prof_mark(c);
-  const size_t shm = 2 * (size_t)ns * sizeof(double);
+  const size_t shm = (fp32 ? 2 : 3) * (size_t)ns * sizeof(double);
   const dim3 g(nr), blk(TAZ_TPB);
-#define TAZ_LAUNCH(K)                                                       \
-  switch (taz_cells(ns)) {                                                  \
-    case 1: hipLaunchKernelGGL(K<1>, g, blk, shm, c->stream, za); break;   \
-    case 2: hipLaunchKernelGGL(K<2>, g, blk, shm, c->stream, za); break;   \
-    case 3: hipLaunchKernelGGL(K<3>, g, blk, shm, c->stream, za); break;   \
-    case 4: hipLaunchKernelGGL(K<4>, g, blk, shm, c->stream, za); break;   \
-    case 6: hipLaunchKernelGGL(K<6>, g, blk, shm, c->stream, za); break;   \
-    case 8: hipLaunchKernelGGL(K<8>, g, blk, shm, c->stream, za); break;   \
-    case 12: hipLaunchKernelGGL(K<12>, g, blk, shm, c->stream, za); break; \
-    default: hipLaunchKernelGGL(K<16>, g, blk, shm, c->stream, za); break; \
+  switch (taz_cells(ns)) {
+    case 1: taz_launch<1>(fp32, g, blk, shm, c->stream, za); break;
+    case 2: taz_launch<2>(fp32, g, blk, shm, c->stream, za); break;
+    case 3: taz_launch<3>(fp32, g, blk, shm, c->stream, za); break;
+    case 4: taz_launch<4>(fp32, g, blk, shm, c->stream, za); break;
+    case 6: taz_launch<6>(fp32, g, blk, shm, c->stream, za); break;
+    case 8: taz_launch<8>(fp32, g, blk, shm, c->stream, za); break;
+    case 12: taz_launch<12>(fp32, g, blk, shm, c->stream, za); break;
+    default: taz_launch<16>(fp32, g, blk, shm, c->stream, za); break;
   }
-  if (fp32) {
-    TAZ_LAUNCH(k_taz32)
-  } else {
-    TAZ_LAUNCH(k_taz)
-  }
-#undef TAZ_LAUNCH
   FG_LAUNCH_CHECK();
   // k_fin
   fin_args fa;
@@ -1602,7 +1620,7 @@ __global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *__restr
   extern __shared__ double lds_btaz[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) taz32_ring<C>(D[w.x].taz, w.y, lds_btaz);
+    if (D[w.x].active) taz32_ring<C, false>(D[w.x].taz, w.y, lds_btaz);
     __syncthreads();
   }
 }

@@ -292,6 +292,7 @@ struct fg_sim {
   double IndirectX = 0.0, IndirectY = 0.0;
   int fused_active = 0;   // substeps run through fg_step_fused
   long nsub = 0;          // substeps taken since creation
+  int planets_fresh = 0;  // planetK.dat emptied (first output) or kept (restart)
   int results_ready = 0;  // fused CFL count / star force are current
 };
 
@@ -943,17 +944,13 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
     return st;
   }
   if (s->opt.write_files) {
-    // WriteDim (Output.cu:19-35), EmptyPlanetSystemFile (Output.cu:37)
+    // WriteDim (Output.cu:19-35); the planet files are emptied by the first
+    // output (EmptyPlanetSystemFile, Output.cu:37), not after a restart
     std::string path = d.OUTPUTDIR + "dims.dat";
     FILE *f = fopen(path.c_str(), "w");
     if (f) {
       fprintf(f, "%d\t%d\t\t%d\t%d\t%f\t%d\t%d\t%d\n", 0, 0, 0, 0, d.RMAX, d.NTOT / d.NINTERM, nr, ns);
       fclose(f);
-    }
-    for (int k = 0; k < s->ps.nb; k++) {
-      path = d.OUTPUTDIR + "planet" + std::to_string(k) + ".dat";
-      f = fopen(path.c_str(), "w");
-      if (f) fclose(f);
     }
   }
   s->PhysicalTimeInitial = s->PhysicalTime;
@@ -1187,6 +1184,14 @@ extern "C" fg_status fg_sim_write_outputs(fg_sim *s, int idx) {
     if (d.Write_Temperature) CHECK(write_polar(s, temp, "temperature", idx));
     if (d.Write_Qplus) CHECK(write_polar(s, qp, "qplus", idx));
   }
+  if (!s->planets_fresh) {  // EmptyPlanetSystemFile, Output.cu:37-59
+    for (int k = 0; k < s->ps.nb; k++) {
+      std::string path = d.OUTPUTDIR + "planet" + std::to_string(k) + ".dat";
+      FILE *f = fopen(path.c_str(), "w");
+      if (f) fclose(f);
+    }
+    s->planets_fresh = 1;
+  }
   for (int k = 0; k < s->ps.nb; k++) {
     std::string path = d.OUTPUTDIR + "planet" + std::to_string(k) + ".dat";
     FILE *f = fopen(path.c_str(), "a");
@@ -1200,6 +1205,89 @@ extern "C" fg_status fg_sim_write_outputs(fg_sim *s, int idx) {
     fclose(f);
   }
   return FG_OK;
+}
+
+// Restart from output set `idx`: the reference parses -s N (Main.cu:152-160)
+// but never reads the files back.  The fields are the raw NRAD*NSEC FP64
+// dumps of WriteDiskPolar (Output.cu:150-181), the planets, time, frame
+// rate and lost mass the line with TimeStep == idx of each planetK.dat
+// (Output.cu:110-131, %.18g: an exact round trip).  Outputs are written at
+// DT boundaries, so no sub-cycle state is carried.
+static fg_status read_polar(fg_sim *s, std::vector<double> &a, const char *name, int idx) {
+  std::string path = s->d.OUTPUTDIR + name + "/" + name + std::to_string(idx) + ".dat";
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) {
+    host_error("restart: cannot open '%s'", path.c_str());
+    return FG_E_IO;
+  }
+  const size_t n = (size_t)s->nr * s->ns;
+  const size_t got = fread(a.data(), sizeof(double), n, f);
+  fclose(f);
+  if (got != n) {
+    host_error("restart: '%s' holds %zu of %zu values", path.c_str(), got, n);
+    return FG_E_IO;
+  }
+  return FG_OK;
+}
+
+extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
+  if (!s || idx < 0) return FG_E_INVALID;
+  const Deck &d = s->d;
+  std::vector<double> dens(s->S), vr(s->S), vt(s->S), en(s->S, 0.0);
+  // current state first: the ghost ring is not in the files
+  CHECK(fg_download_state(s->ctx, dens.data(), vr.data(), vt.data(), d.Adiabatic ? en.data() : nullptr, nullptr,
+                          nullptr));
+  CHECK(read_polar(s, dens, "dens", idx));
+  CHECK(read_polar(s, vr, "vrad", idx));
+  CHECK(read_polar(s, vt, "vtheta", idx));
+  if (d.Adiabatic) CHECK(read_polar(s, en, "energy", idx));
+  double t = idx * d.NINTERM * d.DT, omega = s->OmegaFrame, lost = s->LostMass;
+  Planets &p = s->ps;
+  for (int k = 0; k < p.nb; k++) {
+    std::string path = d.OUTPUTDIR + "planet" + std::to_string(k) + ".dat";
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) {
+      host_error("restart: cannot open '%s'", path.c_str());
+      return FG_E_IO;
+    }
+    char line[1024];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+      int n = 0;
+      double v[10];
+      if (sscanf(line, "%d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &n, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                 &v[6], &v[7], &v[8], &v[9]) >= 9 &&
+          n == idx) {
+        p.x[k] = v[0];
+        p.y[k] = v[1];
+        p.vx[k] = v[2];
+        p.vy[k] = v[3];
+        p.mass[k] = v[4];
+        lost = v[5];
+        t = v[6];
+        omega = v[7];
+        found = true;
+      }
+    }
+    fclose(f);
+    if (!found) {
+      host_error("restart: no line %d in '%s'", idx, path.c_str());
+      return FG_E_IO;
+    }
+  }
+  s->PhysicalTime = t;
+  s->prof.time = t;
+  s->OmegaFrame = omega;
+  s->LostMass = lost;
+  s->in_dt = 0;
+  s->results_ready = 0;
+  s->planets_fresh = 1;  // keep the planet files: their lines up to idx are this run's history
+  CHECK(fg_upload_state(s->ctx, dens.data(), vr.data(), vt.data(), d.Adiabatic ? en.data() : nullptr));
+  CHECK(fg_sound_speed(s->ctx));
+  CHECK(fg_pressure(s->ctx));
+  CHECK(fg_temperature(s->ctx));
+  if (d.ViscosityAlpha) CHECK(fg_ring_mean_cs(s->ctx));
+  return fg_sync(s->ctx);
 }
 
 // main()'s loop, Main.cu:309-333

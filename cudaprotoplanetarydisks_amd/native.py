@@ -90,6 +90,7 @@ SIGNATURES = {
     "fg_sim_algogas": (ctypes.c_int, [_vp]),
     "fg_sim_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "fg_sim_write_outputs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fg_sim_restart": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fg_sim_get_state": (ctypes.c_int, [_vp, _pd, _pd, _pd, _pd]),
     "fg_sim_set_state": (ctypes.c_int, [_vp, _pd, _pd, _pd, _pd]),
     "fg_sim_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _pd]),

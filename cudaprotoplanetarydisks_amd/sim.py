@@ -39,6 +39,11 @@ class Simulation:
         self.h = h
         self.nrad = int(self.get("NRAD"))
         self.nsec = int(self.get("NSEC"))
+        if precision == "fp32":  # autotuned tile shapes of this grid, if any
+            from .ensemble import load_tiles
+            t = load_tiles().get(f"{self.nrad}x{self.nsec}")
+            if t:
+                call("fg_set_tiles", lib().fg_sim_ctx(self.h), int(t["src_rows"]), int(t["trad_band"]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -76,6 +81,10 @@ class Simulation:
 
     def write_outputs(self, index):
         call("fg_sim_write_outputs", self.h, index)
+
+    def restart(self, index):
+        """Continue from output set `index` of this deck's output directory."""
+        call("fg_sim_restart", self.h, index)
 
     def state(self, energy=False):
         out = [np.empty(self.shape) for _ in range(4 if energy else 3)]

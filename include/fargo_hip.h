@@ -247,6 +247,10 @@ fg_status fg_sim_algogas(fg_sim *sim);
 /* main()'s loop for it = it0..it1 (Main.cu:309-333), with outputs */
 fg_status fg_sim_run(fg_sim *sim, int it0, int it1);
 fg_status fg_sim_write_outputs(fg_sim *sim, int index);
+/* restart from output set `index` (dens/densN.dat, vrad, vtheta, energy
+ * when adiabatic, and line N of every planetK.dat): what -s N
+ * (Main.cu:152-160) announces; the reference never reads them back */
+fg_status fg_sim_restart(fg_sim *sim, int index);
 fg_status fg_sim_get_state(fg_sim *sim, double *dens, double *vrad, double *vtheta, double *energy);
 fg_status fg_sim_set_state(fg_sim *sim, const double *dens, const double *vrad, const double *vtheta,
                            const double *energy);
