@@ -153,9 +153,11 @@ def main():
         d, _, _ = sim.state()
         finite = bool(np.isfinite(d).all())
         sim.close()
-        word = 8 if args.precision == "fp64" else 4
+        # both precisions keep the state (Sigma, vrad, vtheta) and the
+        # quantities passed between kernels in FP64; FP32 is the arithmetic
+        word = 8
         zus = nr * ns * args.steps / elapsed
-        b_alg = 48 if word == 8 else 24
+        b_alg = 48  # read + write of Sigma, vrad, vtheta per zone-update
         roofline = None
         if prof:
             dom = max(prof, key=prof.get)
@@ -183,10 +185,11 @@ def main():
         "metric": METRIC, "value": round(zus, 1), "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if word == 8 else "f32", "data": "synthetic",
+        "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
         "config": {"workload": f"plain FARGO substep, {nr}x{ns} log grid, isothermal h=0.05, nu=1e-5, "
                                "q=1e-3 planet, evanescent edges, seeded 1e-4 noise on Sigma",
-                   "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision},
+                   "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision,
+                   "state": "fp64"},
         "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
         "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,

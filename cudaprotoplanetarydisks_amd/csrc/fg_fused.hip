@@ -588,6 +588,7 @@ struct fin_args {
   const double *xd, *xvt, *xrmp, *xrmm, *press;
   double *dens, *vr, *vt, *temp;
   double *vmoy, *newdt;
+  double *fpart;  // FP32 mode: per-ring compensated star-force partials (4 per ring), else NULL
   const double *nu;
   fg_dev_tables t;
   double dt, omega_frame;
@@ -602,6 +603,63 @@ __device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
   return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
 }
 
+// Neumaier-compensated sums (value s, correction c) of the FP32 mode's
+// star force: per thread in cell order, then a fixed shuffle tree per
+// wave and the waves in order
+__device__ __forceinline__ void kadd(double &s, double &c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+__device__ __forceinline__ void kmerge(double &s, double &c, double s2, double c2) {
+  kadd(s, c, s2);
+  c += c2;
+}
+__device__ __forceinline__ void kwave(double &s, double &c) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double s2 = __shfl_down(s, off), c2 = __shfl_down(c, off);
+    kmerge(s, c, s2, c2);
+  }
+}
+
+// disk force on the star from the per-ring partials, in a fixed order: each
+// thread merges a contiguous run of rings, then a shuffle tree per wave,
+// then the waves in order.  Called by every thread of a 256-thread block.
+__device__ __forceinline__ void force_rings_final(const double *fpart, int nr, double *out2) {
+  __shared__ double ws[4][4];
+  const int tid = threadIdx.x, per = (nr + 255) / 256;
+  double sx = 0.0, cx = 0.0, sy = 0.0, cy = 0.0;
+  for (int q = 0; q < per; q++) {
+    const int i = tid * per + q;
+    if (i < nr) {
+      kmerge(sx, cx, fpart[4 * i], fpart[4 * i + 1]);
+      kmerge(sy, cy, fpart[4 * i + 2], fpart[4 * i + 3]);
+    }
+  }
+  kwave(sx, cx);
+  kwave(sy, cy);
+  if ((tid & 63) == 0) {
+    ws[tid >> 6][0] = sx;
+    ws[tid >> 6][1] = cx;
+    ws[tid >> 6][2] = sy;
+    ws[tid >> 6][3] = cy;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = ws[0][0], b = ws[0][1], c = ws[0][2], d = ws[0][3];
+    for (int w = 1; w < 4; w++) {
+      kmerge(a, b, ws[w][0], ws[w][1]);
+      kmerge(c, d, ws[w][2], ws[w][3]);
+    }
+    out2[0] = a + b;
+    out2[1] = c + d;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr, double *out2) {
+  force_rings_final(fpart, nr, out2);
+}
+
 // one ring; ring holds NSEC doubles (the ring's final vtheta)
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[256];
@@ -614,6 +672,8 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
   if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i);
   if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
+  double fsx = 0.0, fcx = 0.0, fsy = 0.0, fcy = 0.0;
+  const double Rm = t.Rmed[i], surf = t.Surf[i];
   for (int j = tid; j < ns; j += 256) {
     long l = base + j;
     double vr = recover_vr(a, i, j), vt = a.xvt[l], s = a.xd[l];
@@ -621,6 +681,14 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       vr = damp_toward(vr, lam_i, vr0_i);
       vt = damp_toward(vt, lam_i, vt0_i);
       s = damp_toward(s, lam_i, t.SigmaMed[i]);
+    }
+    if (a.fpart) {  // gravity of the cell on the star (Kernels.cu:703-727 at x = y = 0)
+      const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
+      const double dist2 = dx * dx + dy * dy;
+      const double inv3 = 1.0 / dist2 / sqrt(dist2);
+      const double cm = surf * s;
+      kadd(fsx, fcx, cm * dx * inv3);
+      kadd(fsy, fcy, cm * dy * inv3);
     }
     a.dens[l] = s;
     a.vr[l] = vr;
@@ -631,6 +699,29 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       a.dens[l + ns] = 0.0;
       a.vr[l + ns] = 0.0;
       a.vt[l + ns] = 0.0;
+    }
+  }
+  if (a.fpart) {
+    __shared__ double fw[4][4];
+    kwave(fsx, fcx);
+    kwave(fsy, fcy);
+    if ((tid & 63) == 0) {
+      fw[tid >> 6][0] = fsx;
+      fw[tid >> 6][1] = fcx;
+      fw[tid >> 6][2] = fsy;
+      fw[tid >> 6][3] = fcy;
+    }
+    __syncthreads();
+    if (tid == 64) {  // a lane of another wave than the ring sum below
+      double sx = fw[0][0], cx = fw[0][1], sy = fw[0][2], cy = fw[0][3];
+      for (int w = 1; w < 4; w++) {
+        kmerge(sx, cx, fw[w][0], fw[w][1]);
+        kmerge(sy, cy, fw[w][2], fw[w][3]);
+      }
+      a.fpart[4 * i] = sx;
+      a.fpart[4 * i + 1] = cx;
+      a.fpart[4 * i + 2] = sy;
+      a.fpart[4 * i + 3] = cy;
     }
   }
   __syncthreads();
@@ -1280,6 +1371,8 @@ static fg_status fused_alloc(fg_ctx *c) {
   FG_HIP(hipMalloc((void **)&pool, 14 * S * sizeof(double)));
   FG_HIP(hipMemset(pool, 0, 14 * S * sizeof(double)));
   c->fu_ring = pool;
+  FG_HIP(hipMalloc((void **)&c->fu_fpart, 4 * (size_t)(c->nr + 1) * sizeof(double)));
+  FG_HIP(hipMemset(c->fu_fpart, 0, 4 * (size_t)(c->nr + 1) * sizeof(double)));
   return FG_OK;
 }
 
@@ -1318,8 +1411,15 @@ static fg_status fused_supported(fg_ctx *c) {
 
 // results of the reductions for the next substep: CFL count and the disk
 // force on the star; copied to pinned memory behind an event
-static fg_status fused_reductions(fg_ctx *c, double DeltaT) {
-  if (fg_status s = fg_disk_force_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, c->red_out)) return s;
+// ring_partials: the star force comes from k_fin's per-ring compensated
+// partials (FP32 mode) instead of the DeviceReduce-order sum
+static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials = false) {
+  if (ring_partials) {
+    hipLaunchKernelGGL(k_force_rings, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out);
+    FG_LAUNCH_CHECK();
+  } else if (fg_status s = fg_disk_force_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, c->red_out)) {
+    return s;
+  }
   prof_mark(c);
   hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
                      c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
@@ -1547,11 +1647,12 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.ns = ns;
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
+  fa.fpart = fp32 ? c->fu_fpart : nullptr;
   prof_mark(c);
   hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   FG_LAUNCH_CHECK();
   prof_mark(c);
-  return fused_reductions(c, DeltaT_next);
+  return fused_reductions(c, DeltaT_next, fp32);
 }
 
 // ================================================== batched ensemble ====
@@ -1649,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_batch_close(const batch_disk *__restric
   for (int k = blockIdx.x; k < ndisk; k += gridDim.x) {
     const batch_disk &d = D[k];
     if (d.active) {
-      dreduce_final(d.partial, d.B, out, 1, 1);
+      force_rings_final(d.fin.fpart, d.nr, out);
       __syncthreads();
       cfl_final(&res[k].cfl, d.DT1D, d.Vmoy, d.newDT, d.invRmed, d.coef, d.DeltaT, d.nr);
       if (threadIdx.x == 0) {
@@ -1810,7 +1911,6 @@ static fg_status batch_build(fg_batch_dev *b) {
   }
   hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
                      b->it_fin, b->n_fin);
-  hipLaunchKernelGGL(k_batch_force, grid(b->n_force), dim3(RED_TPB), 0, st, b->d_disk, b->it_force, b->n_force);
   hipLaunchKernelGGL(k_batch_close, grid(b->n), dim3(256), 0, st, b->d_disk, b->d_res, b->n);
   (void)hipMemcpyAsync(b->h_res, b->d_res, b->n * sizeof(batch_result), hipMemcpyDeviceToHost, st);
   hipError_t ce = hipStreamEndCapture(st, &b->graph);
@@ -1923,6 +2023,7 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fa.ns = ns;
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
+  fa.fpart = c->fu_fpart;
   d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
   d.ncell = (long)nr * ns;
   d.B = red_blocks(d.ncell);

@@ -106,6 +106,7 @@ struct fg_ctx {
   // fused-path scratch
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results
+  double *fu_fpart;   // FP32 mode: per-ring star-force partials (4 * NRAD)
 };
 
 static inline int fg_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }

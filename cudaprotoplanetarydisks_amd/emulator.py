@@ -177,7 +177,7 @@ def orbit_history(ens, max_orbits, log=None, label=""):
     """Advance every disk max_orbits orbits; per orbit, the orbit average
     (over its 20 DT boundaries) of each disk's ring-averaged Sigma / Sigma_init
     and its steadiness against the previous orbit.  Returns R[disk][orbit],
-    the last orbit averages and the wall time."""
+    the orbit averages hist[disk][orbit - 1] and the wall time."""
     n = ens.n
     per_orbit = int(round(ORBIT / DT))
     rmed = [ens.ring(k, "Rmed")[:ens.shapes[k][0] - 1] for k in range(n)]
@@ -186,6 +186,7 @@ def orbit_history(ens, max_orbits, log=None, label=""):
     sigmed = [ens.ring(k, "SigmaMed")[:ens.shapes[k][0] - 1] for k in range(n)]
     prev = None
     R = [[] for _ in range(n)]
+    hist = [[] for _ in range(n)]  # orbit averages, orbit 1..max_orbits
     avg = None
     for o in range(1, max_orbits + 1):
         acc = [0.0] * n
@@ -194,6 +195,8 @@ def orbit_history(ens, max_orbits, log=None, label=""):
             for k in range(n):
                 acc[k] = acc[k] + ens.dens(k)[:-1].mean(axis=1) / sigmed[k]
         avg = [a / per_orbit for a in acc]
+        for k in range(n):
+            hist[k].append(avg[k])
         if prev is not None:
             for k in range(n):
                 R[k].append(steadiness(prev[k], avg[k], rmed[k]))
@@ -201,7 +204,7 @@ def orbit_history(ens, max_orbits, log=None, label=""):
         if log and o % 25 == 0:
             log({"start": label, "orbit": o, "R_median": float(np.median([r[-1] for r in R])),
                  "seconds": round(time.perf_counter() - t0, 1)})
-    return R, avg, time.perf_counter() - t0
+    return R, hist, time.perf_counter() - t0
 
 
 def first_below(r, eps):
@@ -258,7 +261,14 @@ def study(n_train, n_eval, nrad, orbits, steps, eps, max_orbits, seed, log=print
         tc = [first_below(r, e) for r in R_cold]
         tw = [first_below(r, e) for r in R_warm]
         both = [k for k in range(n_eval) if tc[k] and tw[k]]
+        # profiles at the orbit where each start first meets the criterion
+        steady_diff = []
+        for k in both:
+            pc, pw = f_cold[k][tc[k] - 1], f_warm[k][tw[k] - 1]
+            steady_diff.append(float(np.abs(pw - pc).sum() / np.abs(pc).sum()))
         per_eps[str(e)] = {
+            "steady_profile_l1_median": float(np.median(steady_diff)) if steady_diff else None,
+            "steady_profile_within_1pct": int(sum(x <= 0.01 for x in steady_diff)),
             "reached_cold": int(sum(t is not None for t in tc)), "reached_warm": int(sum(t is not None for t in tw)),
             "orbits_cold": tc, "orbits_warm": tw,
             "median_speedup": float(np.median([tc[k] / tw[k] for k in both])) if both else None,
@@ -268,8 +278,9 @@ def study(n_train, n_eval, nrad, orbits, steps, eps, max_orbits, seed, log=print
     report["time_to_steady"] = per_eps
     diffs = []
     for k in range(n_eval):
-        d = np.abs(f_warm[k] - f_cold[k])
-        diffs.append({"l1": float(d.sum() / np.abs(f_cold[k]).sum()), "linf": float((d / np.abs(f_cold[k])).max())})
+        d = np.abs(f_warm[k][-1] - f_cold[k][-1])
+        diffs.append({"l1": float(d.sum() / np.abs(f_cold[k][-1]).sum()),
+                      "linf": float((d / np.abs(f_cold[k][-1])).max())})
     report["final_profile_diff"] = {"l1_median": float(np.median([x["l1"] for x in diffs])),
                                     "l1_max": float(max(x["l1"] for x in diffs)),
                                     "within_1pct_l1": int(sum(x["l1"] <= 0.01 for x in diffs)),

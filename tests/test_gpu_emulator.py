@@ -55,8 +55,9 @@ def test_warm_state_is_a_valid_restart(tmp_path):
 def test_steadiness_history_and_thresholds(tmp_path):
     ps = em.draw(3, 4, 24)
     with em.open_ensemble(ps, str(tmp_path)) as ens:
-        R, avg, wall = em.orbit_history(ens, 4)
+        R, hist, wall = em.orbit_history(ens, 4)
     assert len(R) == 3 and all(len(r) == 3 for r in R)
+    assert all(len(h) == 4 for h in hist)
     assert all(np.isfinite(x) and x >= 0 for r in R for x in r)
     assert em.first_below([0.5, 0.2, 0.05], 0.1) == 4
     assert em.first_below([0.5, 0.2], 0.1) is None
