@@ -19,6 +19,7 @@ static void usage(const char *exe) {  // PrintUsage, Interpret.cu:216-239
   printf("-s N : Restart from output set N of the output directory\n");
   printf("-F : (this build) use the fused substep kernels\n");
   printf("-S : (this build) FP32 bulk precision mode\n");
+  printf("-L : (this build) circumplanetary mass, bigplanetK.dat and tqwkK*.dat torque logs\n");
   exit(1);
 }
 
@@ -34,14 +35,15 @@ int main(int argc, char **argv) {
   }
   if (argc == 1) usage(argv[0]);
   std::string flags, outdir, parfile;
-  int disable = 0, fused = 0, fp32 = 0, restart = -1;
+  int disable = 0, fused = 0, fp32 = 0, restart = -1, diagnostics = 0;
   for (int i = 1; i < argc; i++) {
     if (argv[i][0] == '-') {
-      if (strspn(argv[i], "-secndovtpfamzib0123456789FS") != strlen(argv[i])) usage(argv[0]);
+      if (strspn(argv[i], "-secndovtpfamzib0123456789FSL") != strlen(argv[i])) usage(argv[0]);
       flags += argv[i] + 1;
       if (strchr(argv[i], 'n')) disable = 1;
       if (strchr(argv[i], 'F')) fused = 1;
       if (strchr(argv[i], 'S')) fp32 = 1;
+      if (strchr(argv[i], 'L')) diagnostics = 1;
       if (strchr(argv[i], 's')) {  // Main.cu:152-160
         if (++i >= argc) usage(argv[0]);
         restart = atoi(argv[i]);
@@ -63,6 +65,7 @@ int main(int argc, char **argv) {
   opt.write_files = disable ? 0 : 1;
   opt.outdir = outdir.empty() ? nullptr : outdir.c_str();
   opt.cli_flags = flags.c_str();
+  opt.diagnostics = diagnostics;
   fg_sim *sim = nullptr;
   fg_status st = fg_sim_create(parfile.c_str(), &opt, &sim);
   if (st) {

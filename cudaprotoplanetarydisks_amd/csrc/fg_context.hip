@@ -235,7 +235,8 @@ extern "C" fg_status fg_destroy(fg_ctx *c) {
                   c->DT1D,         c->VMed,         c->nu,            c->cs_mean,        c->sgacc_mean,
                   c->Nshift,       c->NoSplit,      c->LostByDisk,    c->red_partial,    c->red_out,
                   c->cfl_out,      c->sg_Sr,        c->sg_St,         c->sg_Krc,         c->sg_Ktc,
-                  c->sg_Gr,        c->sg_Gt,        c->sg_tmp,        c->fu_ring,        c->fu_fpart};
+                  c->sg_Gr,        c->sg_Gt,        c->sg_tmp,        c->fu_ring,        c->fu_fpart,
+                  c->hill_partial, c->hill_out};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int k = 0; k < FG_NFIELDS; k++)

@@ -287,6 +287,7 @@ struct fg_sim {
   // scalars of the run
   double PhysicalTime = 0.0, PhysicalTimeInitial = 0.0, OmegaFrame = 0.0, MassTaper = 0.0, LostMass = 0.0;
   double HillRadius = 0.0;
+  double mdcp = 0.0, mdcp1 = 0.0, exces_mdcp = 0.0;  // circumplanetary mass (diagnostics)
   double dt = 0.0, dtemp = 0.0;
   int in_dt = 0, last_cfl = 0;
   double IndirectX = 0.0, IndirectY = 0.0;
@@ -565,6 +566,22 @@ void advance_rk5(fg_sim *s, double dt) {
     p.vx[0] = vx * cos(dtheta + theta) - vy * sin(dtheta + theta);
     p.vy[0] = vx * sin(dtheta + theta) + vx * cos(dtheta + theta);  // as Pframeforce.cu:184
   }
+}
+
+// CircumPlanetaryMass, SourceEuler.cu:549-563: around planet 0, with the
+// Hill radius of the initial planet system (Psys.cu:140)
+static fg_status circumplanetary_mass(fg_sim *s, double *m) {
+  *m = 0.0;
+  if (s->ps.nb < 1) return FG_OK;
+  return fg_circumplanetary_mass(s->ctx, s->ps.x[0], s->ps.y[0], s->HillRadius, m);
+}
+
+// end of a substep with diagnostics on, SourceEuler.cu:297-298
+static fg_status substep_diagnostics(fg_sim *s) {
+  if (!s->opt.diagnostics || !s->d.IsDisk) return FG_OK;
+  fg_status st = circumplanetary_mass(s, &s->mdcp1);
+  s->exces_mdcp = s->mdcp1 - s->mdcp;
+  return st;
 }
 
 // smoothing length of a planet's potential / force, Pframeforce.cu:74-76
@@ -968,6 +985,11 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
     return FG_E_UNSUPPORTED;
   }
   s->fused_active = (s->opt.fused || s->opt.precision == FG_PREC_FP32) && fusable;
+  // Main.cu:302: circumplanetary mass of the initial density
+  if (s->opt.diagnostics && (st = circumplanetary_mass(s, &s->mdcp))) {
+    fg_sim_destroy(s);
+    return st;
+  }
   *out = s;
   return FG_OK;
 }
@@ -1063,6 +1085,7 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
   CHECK(fused_plan(s, cfl, force, L));
   CHECK(fg_step_fused(c, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY, L.DeltaT_next,
                       d.Write_Temperature));
+  CHECK(substep_diagnostics(s));
   fused_close(s, L, closed);
   return FG_OK;
 }
@@ -1128,6 +1151,7 @@ extern "C" fg_status fg_sim_substep(fg_sim *s, int *closed) {
     s->LostMass += lost;
     CHECK(fg_boundary(c, dt, s->OmegaFrame));
     CHECK(fg_temperature(c));
+    CHECK(substep_diagnostics(s));
   }
   s->nsub++;
   s->PhysicalTime += dt;
@@ -1201,8 +1225,74 @@ extern "C" fg_status fg_sim_write_outputs(fg_sim *s, int idx) {
     }
     const Planets &p = s->ps;
     fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", idx, p.x[k], p.y[k],
-            p.vx[k], p.vy[k], p.mass[k], s->LostMass, s->PhysicalTime, s->OmegaFrame, 0.0, 0.0);
+            p.vx[k], p.vy[k], p.mass[k], s->LostMass, s->PhysicalTime, s->OmegaFrame, s->mdcp, s->exces_mdcp);
     fclose(f);
+  }
+  return FG_OK;
+}
+
+// WriteBigPlanetSystemFile, Output.cu:62-107: appended every main-loop
+// iteration (Main.cu:314), with the latest circumplanetary mass
+static fg_status write_bigplanet(fg_sim *s, int TimeStep) {
+  const Planets &p = s->ps;
+  for (int k = 0; k < p.nb; k++) {
+    std::string path = s->d.OUTPUTDIR + "bigplanet" + std::to_string(k) + ".dat";
+    FILE *f = fopen(path.c_str(), "a");
+    if (!f) {
+      host_error("Can't write 'bigplanet.dat' file.");
+      return FG_E_IO;
+    }
+    fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep, p.x[k],
+            p.y[k], p.vx[k], p.vy[k], p.mass[k], s->LostMass, s->PhysicalTime, s->OmegaFrame, s->mdcp1,
+            s->exces_mdcp);
+    fclose(f);
+  }
+  return FG_OK;
+}
+
+// UpdateLog, Force.cu:19-84: per planet, the disk torque and power split
+// inside / outside the planet's orbit, with (ex_) and without the Hill cut,
+// into tqwkK.dat; without self-gravity also each of the dimfxy Hill-cut
+// variants into tqwkK_k.dat.  Appended every main-loop iteration
+// (Main.cu:315).
+static fg_status update_log(fg_sim *s, int TimeStep) {
+  const int dimfxy = 11;  // Main.cu:74
+  const Planets &p = s->ps;
+  double gf[4 * dimfxy];
+  for (int i = 0; i < p.nb; i++) {
+    double x = p.x[i], y = p.y[i], vx = p.vx[i], vy = p.vy[i], m = p.mass[i];
+    double r = sqrt(x * x + y * y);
+    double a = sqrt(x * x + y * y);
+    double rh = pow(m / 3., 1. / 3.) * a + 1e-15;
+    double smoothing = planet_smoothing(s, r, m, true);
+    CHECK(fg_disk_force_hill(s->ctx, x, y, smoothing, dimfxy, a, rh, gf));
+    const double fx_inner = gf[0], fx_ex_inner = gf[dimfxy - 1], fx_outer = gf[dimfxy],
+                 fx_ex_outer = gf[2 * dimfxy - 1], fy_inner = gf[2 * dimfxy], fy_ex_inner = gf[3 * dimfxy - 1],
+                 fy_outer = gf[3 * dimfxy], fy_ex_outer = gf[4 * dimfxy - 1];
+    std::string base = s->d.OUTPUTDIR + "tqwk" + std::to_string(i);
+    FILE *f = fopen((base + ".dat").c_str(), "a");
+    if (!f) {
+      host_error("Can't open %s.dat", base.c_str());
+      return FG_E_IO;
+    }
+    fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep,
+            x * fy_inner - y * fx_inner, x * fy_outer - y * fx_outer, x * fy_ex_inner - y * fx_ex_inner,
+            x * fy_ex_outer - y * fx_ex_outer, vx * fx_inner + vy * fy_inner, vx * fx_outer + vy * fy_outer,
+            vx * fx_ex_inner + vy * fy_ex_inner, vx * fx_ex_outer + vy * fy_ex_outer, s->PhysicalTime);
+    fclose(f);
+    if (s->d.SelfGravity) continue;
+    for (int k = 0; k < dimfxy; k++) {
+      std::string path = base + "_" + std::to_string(k) + ".dat";
+      f = fopen(path.c_str(), "a");
+      if (!f) {
+        host_error("Can't open %s", path.c_str());
+        return FG_E_IO;
+      }
+      fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep, x * gf[2 * dimfxy + k] - y * gf[k],
+              x * gf[3 * dimfxy + k] - y * gf[dimfxy + k], vx * gf[k] + vy * gf[2 * dimfxy + k],
+              vx * gf[dimfxy + k] + vy * gf[3 * dimfxy + k], s->PhysicalTime);
+      fclose(f);
+    }
   }
   return FG_OK;
 }
@@ -1241,7 +1331,7 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
   CHECK(read_polar(s, vr, "vrad", idx));
   CHECK(read_polar(s, vt, "vtheta", idx));
   if (d.Adiabatic) CHECK(read_polar(s, en, "energy", idx));
-  double t = idx * d.NINTERM * d.DT, omega = s->OmegaFrame, lost = s->LostMass;
+  double t = idx * d.NINTERM * d.DT, omega = s->OmegaFrame, lost = s->LostMass, mdcp = s->mdcp;
   Planets &p = s->ps;
   for (int k = 0; k < p.nb; k++) {
     std::string path = d.OUTPUTDIR + "planet" + std::to_string(k) + ".dat";
@@ -1254,7 +1344,7 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
     bool found = false;
     while (fgets(line, sizeof line, f)) {
       int n = 0;
-      double v[10];
+      double v[10] = {0.0};
       if (sscanf(line, "%d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &n, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
                  &v[6], &v[7], &v[8], &v[9]) >= 9 &&
           n == idx) {
@@ -1266,6 +1356,7 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
         lost = v[5];
         t = v[6];
         omega = v[7];
+        if (k == 0) mdcp = v[8];
         found = true;
       }
     }
@@ -1287,6 +1378,16 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
   CHECK(fg_pressure(s->ctx));
   CHECK(fg_temperature(s->ctx));
   if (d.ViscosityAlpha) CHECK(fg_ring_mean_cs(s->ctx));
+  if (s->opt.diagnostics) {
+    // the initial circumplanetary mass comes back from the planet file;
+    // the latest one is that of the state just restored (the last substep
+    // before output idx ended there), 0 before the first substep
+    s->mdcp = mdcp;
+    s->mdcp1 = 0.0;
+    if (idx > 0 && d.IsDisk) CHECK(circumplanetary_mass(s, &s->mdcp1));
+    s->exces_mdcp = s->mdcp1 - s->mdcp;
+    if (idx == 0) s->exces_mdcp = 0.0;
+  }
   return fg_sync(s->ctx);
 }
 
@@ -1294,6 +1395,13 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
 extern "C" fg_status fg_sim_run(fg_sim *s, int it0, int it1) {
   if (!s) return FG_E_INVALID;
   for (int i = it0; i <= it1; i++) {
+    if (s->opt.write_files && s->opt.diagnostics) {
+      // Main.cu:312-316 run before TimeStep is updated for iteration i, so
+      // they carry the output number of iteration i - 1 (0 at the start)
+      int prev = i > 0 ? (i - 1) / s->d.NINTERM : 0;
+      CHECK(write_bigplanet(s, prev));
+      CHECK(update_log(s, prev));
+    }
     int TimeStep = i / s->d.NINTERM;
     if (s->opt.write_files && s->d.NINTERM * TimeStep == i) CHECK(fg_sim_write_outputs(s, TimeStep));
     CHECK(fg_sim_algogas(s));
@@ -1327,7 +1435,8 @@ extern "C" fg_status fg_sim_get(fg_sim *s, const char *name, double *v) {
              {"in_dt", (double)s->in_dt},       {"IndirectTerm.x", s->IndirectX},
              {"IndirectTerm.y", s->IndirectY},  {"HillRadius", s->HillRadius},
              {"Adiabatic", (double)s->d.Adiabatic}, {"substeps", (double)s->nsub}, {"SelfGravity", (double)s->d.SelfGravity},
-             {"fused_active", (double)s->fused_active}};
+             {"fused_active", (double)s->fused_active}, {"mdcp", s->mdcp}, {"mdcp1", s->mdcp1},
+             {"exces_mdcp", s->exces_mdcp}};
   for (auto &e : tab)
     if (strcmp(e.k, name) == 0) {
       *v = e.val;
@@ -1415,6 +1524,10 @@ extern "C" fg_status fg_batch_create(int n, const char *const *parfiles, const f
     return FG_E_INVALID;
   }
   *out = nullptr;
+  if (opt && opt->diagnostics) {
+    host_error("fg_batch_create: diagnostics (circumplanetary mass, torque logs) are per-disk runs only");
+    return FG_E_UNSUPPORTED;
+  }
   fg_sim_options o = opt ? *opt : fg_sim_options{};
   o.precision = FG_PREC_FP32;
   o.fused = 1;

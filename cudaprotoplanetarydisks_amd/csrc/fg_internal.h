@@ -14,6 +14,9 @@
 #define FG_CFLSECURITY 0.5
 // DeviceReduce (Kernels.cu:364-394) never uses more than 1024 blocks.
 #define FG_RED_BLOCKS 1024
+// most Hill-cut variants of one fg_disk_force_hill call (the reference uses
+// dimfxy = 11, Main.cu:74)
+#define FG_HILL_MAX 64
 
 void fg_set_error(const char *fmt, ...);
 
@@ -80,7 +83,7 @@ struct fg_ctx {
   double *LostByDisk;  // ns
   // reduction scratch
   double *red_partial;  // 2 * FG_RED_BLOCKS
-  double *red_out;      // [0..1] force, [2] lost mass, [3] spare
+  double *red_out;      // [0..1] force, [2] lost mass, [3] circumplanetary mass, [4..5] ring means
   int *cfl_out;         // [0] CFL count
   double *h_out;        // pinned host mirror of red_out (4 doubles) + CFL
   int *h_cfl;
@@ -107,6 +110,9 @@ struct fg_ctx {
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results
   double *fu_fpart;   // FP32 mode: per-ring star-force partials (4 * NRAD)
+  // Hill-cut force variants (fg_disk_force_hill), allocated on first use
+  double *hill_partial;  // 4 * FG_HILL_MAX * FG_RED_BLOCKS block sums
+  double *hill_out;      // 4 * FG_HILL_MAX results
 };
 
 static inline int fg_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -49,6 +49,56 @@ struct force_src {
   }
 };
 
+// ComputeForceKernel with the Hill cut, Kernels.cu:693-729 (G = 1): the
+// summands of fxi, fxo, fyi, fyo for variant k of dimfxy.  A cell lands in
+// the inner pair when Rmed[i] < a and in the outer pair otherwise; the
+// other pair holds the 0 of the memset arrays (Force.cu:111-114).
+struct hill_force_src {
+  const double *dens, *Surf, *Rmed, *cosc, *sinc;
+  double x, y, rs, a, rh;
+  int ns, dimfxy;
+  __device__ __forceinline__ void operator()(long e, int k, double v[4]) const {
+    int i = (int)(e / ns);
+    int j = (int)(e - (long)i * ns);
+    double cellmass = Surf[i] * dens[e];
+    double dx = Rmed[i] * cosc[j] - x;
+    double dy = Rmed[i] * sinc[j] - y;
+    double d2 = dx * dx + dy * dy;
+    double dist2 = d2 + rs * rs;
+    double distance = sqrt(dist2);
+    double InvDist3 = 1.0 / dist2 / distance;
+    double hill_cut = 1.0;
+    if (k != 0) {
+      double rhk = rh * ((double)k / (double)(dimfxy - 1));
+      hill_cut = 1. - exp(-d2 / (rhk * rhk));
+    }
+    double fx = cellmass * dx * InvDist3 * hill_cut;
+    double fy = cellmass * dy * InvDist3 * hill_cut;
+    const bool inner = Rmed[i] < a;
+    v[0] = inner ? fx : 0.0;
+    v[1] = inner ? 0.0 : fx;
+    v[2] = inner ? fy : 0.0;
+    v[3] = inner ? 0.0 : fy;
+  }
+};
+
+// CircumPlanetaryMassKernel, Kernels.cu:236-251: the mass of cells closer
+// than the Hill radius to (xpl, ypl)
+struct cpm_src {
+  const double *dens, *Surf, *Rmed, *cosc, *sinc;
+  double xpl, ypl, hill;
+  int ns;
+  __device__ __forceinline__ void operator()(long e, double &m, double &zero) const {
+    int i = (int)(e / ns);
+    int j = (int)(e - (long)i * ns);
+    double dx = Rmed[i] * cosc[j] - xpl;
+    double dy = Rmed[i] * sinc[j] - ypl;
+    double dist = sqrt(dx * dx + dy * dy);
+    m = dist < hill ? Surf[i] * dens[e] : 0.0;
+    zero = 0.0;
+  }
+};
+
 struct array_src {
   const double *a, *b;  // b may be NULL
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
@@ -93,6 +143,43 @@ __device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int
     for (int off = 16; off > 0; off >>= 1) acc += __shfl_down(acc, off, 32);
     if (threadIdx.x == 0) partial[blk] = acc;
     if (threadIdx.x == 32) partial[B + blk] = acc;
+  }
+}
+
+// Four DeviceReduce-order sums at once, block blk of B for summand set k:
+// the four outputs are staged in LDS as in dreduce_block, then waves 0 and
+// 1 replay the reference block (lanes 0-31 of wave w add output 2w, lanes
+// 32-63 output 2w + 1).  partial[o * B + blk] receives output o.
+template <class Src>
+__device__ __forceinline__ void dreduce_block4(const Src &src, int k, long n, int B, int blk, double *partial) {
+  __shared__ double v[4][RED_CH][64];
+  const long G = 64L * B;
+  const long b64 = 64L * blk;
+  const int o = threadIdx.x >> 5, t = threadIdx.x & 31;
+  double acc = 0.0;
+  for (long k0 = 0; b64 + k0 * G < n; k0 += RED_CH) {
+    for (int c = threadIdx.x; c < RED_CH * 64; c += RED_TPB) {
+      int kk = c >> 6, lane = c & 63;
+      long e = b64 + (k0 + kk) * G + lane;
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      if (e < n) src(e, k, s);
+      for (int q = 0; q < 4; q++) v[q][kk][lane] = s[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 128) {
+      for (int kk = 0; kk < RED_CH; kk++) {
+        long e = b64 + (k0 + kk) * G + t;
+        if (e < n) {
+          acc += v[o][kk][t];
+          if (e + 32 < n) acc += v[o][kk][t + 32];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 128) {
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_down(acc, off, 32);
+    if (t == 0) partial[o * B + blk] = acc;
   }
 }
 

@@ -45,6 +45,71 @@ extern "C" fg_status fg_disk_force(fg_ctx *c, double x, double y, double rsmooth
   return FG_OK;
 }
 
+// ---- ComputeForce with the Hill cut (Force.cu:102-135) ---------------
+// All dimfxy variants in one launch (blockIdx.y = k), each replaying the
+// four DeviceReduce calls of Force.cu:120-123; the final pass adds each of
+// the 4 * dimfxy block-sum rows in order into the GlobalForce layout
+// [k + o * dimfxy], o = fxi, fxo, fyi, fyo.
+__global__ __launch_bounds__(RED_TPB) void k_hill_partial(hill_force_src src, long n, int B, double *partial) {
+  const int k = blockIdx.y;
+  dreduce_block4(src, k, n, B, blockIdx.x, partial + (long)4 * k * B);
+}
+
+__global__ void k_hill_final(const double *partial, int B, int dimfxy, double *out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;  // row k * 4 + o
+  if (r >= 4 * dimfxy) return;
+  const double *p = partial + (long)r * B;
+  double s = 0.0;
+  for (int b = 0; b < B; b++) s += p[b];
+  out[(r & 3) * dimfxy + (r >> 2)] = s;
+}
+
+extern "C" fg_status fg_disk_force_hill(fg_ctx *c, double x, double y, double rsmoothing, int dimfxy, double a,
+                                        double rh, double *globalforce) {
+  if (!c || !globalforce) {
+    fg_set_error("fg_disk_force_hill: NULL argument");
+    return FG_E_INVALID;
+  }
+  if (dimfxy < 1 || dimfxy > FG_HILL_MAX) {
+    fg_set_error("fg_disk_force_hill: dimfxy = %d outside 1..%d", dimfxy, FG_HILL_MAX);
+    return FG_E_INVALID;
+  }
+  const long n = (long)c->nr * c->ns;
+  const int B = red_blocks(n);
+  if (!c->hill_partial) {
+    FG_HIP(hipMalloc((void **)&c->hill_partial, sizeof(double) * 4 * FG_HILL_MAX * FG_RED_BLOCKS));
+    FG_HIP(hipMalloc((void **)&c->hill_out, sizeof(double) * 4 * FG_HILL_MAX));
+  }
+  hill_force_src src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, x, y, rsmoothing, a, rh,
+                     c->ns, dimfxy};
+  hipLaunchKernelGGL(k_hill_partial, dim3(B, dimfxy), dim3(RED_TPB), 0, c->stream, src, n, B, c->hill_partial);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_hill_final, dim3((4 * dimfxy + 63) / 64), dim3(64), 0, c->stream, c->hill_partial, B, dimfxy,
+                     c->hill_out);
+  FG_LAUNCH_CHECK();
+  FG_HIP(hipMemcpyAsync(globalforce, c->hill_out, sizeof(double) * 4 * dimfxy, hipMemcpyDeviceToHost, c->stream));
+  FG_HIP(hipStreamSynchronize(c->stream));
+  return FG_OK;
+}
+
+// ---- CircumPlanetaryMass (SourceEuler.cu:549-563) --------------------
+fg_status fg_circumplanetary_mass_async(fg_ctx *c, double xpl, double ypl, double hill_radius, double *out_dev) {
+  cpm_src src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, xpl, ypl, hill_radius, c->ns};
+  return dreduce(c, src, (long)c->nr * c->ns, out_dev, 0, 0);
+}
+
+extern "C" fg_status fg_circumplanetary_mass(fg_ctx *c, double xpl, double ypl, double hill_radius, double *out) {
+  if (!c || !out) {
+    fg_set_error("fg_circumplanetary_mass: NULL argument");
+    return FG_E_INVALID;
+  }
+  if (fg_status s = fg_circumplanetary_mass_async(c, xpl, ypl, hill_radius, c->red_out + 3)) return s;
+  FG_HIP(hipMemcpyAsync(c->h_out + 3, c->red_out + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FG_HIP(hipStreamSynchronize(c->stream));
+  *out = c->h_out[3];
+  return FG_OK;
+}
+
 // ---- non-reflecting boundary (SideEuler.cu:45-100) -------------------
 // Ring gathers and their DeviceReduce means, then the two shifted-copy
 // kernels and the mean correction.

@@ -47,7 +47,7 @@ class FgSimOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int), ("fused", ctypes.c_int),
                 ("write_files", ctypes.c_int), ("outdir", ctypes.c_char_p),
                 ("cli_flags", ctypes.c_char_p), ("noise_seed", ctypes.c_ulonglong),
-                ("noise_amp", _d)]
+                ("noise_amp", _d), ("diagnostics", ctypes.c_int)]
 
 
 # exported symbols with (restype, argtypes); every one returns fg_status
@@ -69,6 +69,8 @@ SIGNATURES = {
     "fg_ring_mean_cs": (ctypes.c_int, [_vp]),
     "fg_cfl": (ctypes.c_int, [_vp, _d, _pi]),
     "fg_disk_force": (ctypes.c_int, [_vp, _d, _d, _d, _pd]),
+    "fg_disk_force_hill": (ctypes.c_int, [_vp, _d, _d, _d, ctypes.c_int, _d, _d, _pd]),
+    "fg_circumplanetary_mass": (ctypes.c_int, [_vp, _d, _d, _d, _pd]),
     "fg_potential": (ctypes.c_int, [_vp, ctypes.POINTER(FgPlanetPot), ctypes.c_int, _d, _d]),
     "fg_correct_vtheta": (ctypes.c_int, [_vp, _d]),
     "fg_boundary": (ctypes.c_int, [_vp, _d, _d]),

@@ -22,7 +22,7 @@ def _ptr(a):
 
 class Simulation:
     def __init__(self, parfile, device=0, precision="fp64", fused=False, write_files=False, outdir=None,
-                 cli_flags="", noise_seed=0, noise_amp=0.0):
+                 cli_flags="", noise_seed=0, noise_amp=0.0, diagnostics=False):
         opt = native.FgSimOptions()
         opt.device = device
         opt.precision = native.FG_PREC_FP32 if precision == "fp32" else native.FG_PREC_FP64
@@ -34,6 +34,7 @@ class Simulation:
         opt.cli_flags = self._flags
         opt.noise_seed = noise_seed
         opt.noise_amp = noise_amp
+        opt.diagnostics = 1 if diagnostics else 0
         h = ctypes.c_void_p()
         call("fg_sim_create", os.fsencode(parfile), ctypes.byref(opt), ctypes.byref(h))
         self.h = h
@@ -150,6 +151,18 @@ class Context:
         out = np.empty(2)
         call("fg_disk_force", self.h, x, y, rs, _ptr(out))
         return out
+
+    def disk_force_hill(self, x, y, rs, dimfxy, a, rh):
+        """ComputeForce (Force.cu:102-135): the 4*dimfxy GlobalForce sums."""
+        out = np.empty(4 * dimfxy)
+        call("fg_disk_force_hill", self.h, x, y, rs, int(dimfxy), a, rh, _ptr(out))
+        return out
+
+    def circumplanetary_mass(self, xpl, ypl, hill_radius):
+        """CircumPlanetaryMass (SourceEuler.cu:549-563)."""
+        out = ctypes.c_double()
+        call("fg_circumplanetary_mass", self.h, xpl, ypl, hill_radius, ctypes.byref(out))
+        return out.value
 
     def potential(self, planets, ix, iy):
         arr = (native.FgPlanetPot * max(1, len(planets)))()

@@ -167,6 +167,19 @@ fg_status fg_cfl(fg_ctx *ctx, double DeltaT, int *ncfl);
  * mass at (x, y) with smoothing rsmoothing, summed in DeviceReduce order
  * (Kernels.cu:364).  out2 = {ax, ay}. */
 fg_status fg_disk_force(fg_ctx *ctx, double x, double y, double rsmoothing, double *out2);
+/* ComputeForce, Force.cu:102-135 (ComputeForceKernel Kernels.cu:693-729):
+ * disk gravity on a point mass at (x, y), split into cells inside
+ * (Rmed < a) and outside r = a, for the dimfxy Hill-cut variants
+ * k = 0..dimfxy-1 (weight 1 - exp(-d^2 / (rh k/(dimfxy-1))^2), none for
+ * k = 0), each of the 4 dimfxy sums in DeviceReduce order.  globalforce
+ * receives the reference's GlobalForce layout: [k] fx inner,
+ * [dimfxy + k] fx outer, [2 dimfxy + k] fy inner, [3 dimfxy + k] fy outer.
+ * 1 <= dimfxy <= 64.  Used by UpdateLog (Force.cu:19). */
+fg_status fg_disk_force_hill(fg_ctx *ctx, double x, double y, double rsmoothing, int dimfxy, double a, double rh,
+                             double *globalforce);
+/* CircumPlanetaryMass, SourceEuler.cu:549-563 (Kernels.cu:236-251): disk
+ * mass within hill_radius of (xpl, ypl), in DeviceReduce order */
+fg_status fg_circumplanetary_mass(fg_ctx *ctx, double xpl, double ypl, double hill_radius, double *mass);
 /* FillForcesArrays, Pframeforce.cu:56 (+ ComputeIndirectTerm :86) */
 fg_status fg_potential(fg_ctx *ctx, const fg_planet_pot *planets, int nb, double indirect_x,
                        double indirect_y);
@@ -234,6 +247,12 @@ typedef struct fg_sim_options {
   const char *cli_flags; /* single-letter switches of Main.cu:113-179 */
   unsigned long long noise_seed; /* 0 = none; else Sigma *= 1 + amp*xi */
   double noise_amp;
+  /* 1 = the diagnostics the reference ships commented out: the initial
+   * circumplanetary mass (Main.cu:302), its value after every substep
+   * (SourceEuler.cu:297-298, written to the planet files), and per main
+   * loop iteration bigplanetK.dat (Main.cu:314, Output.cu:90-107) and the
+   * torque/power logs tqwkK.dat, tqwkK_k.dat (Main.cu:315, Force.cu:19-84) */
+  int diagnostics;
 } fg_sim_options;
 
 /* Main.cu:187-305 */
@@ -255,7 +274,8 @@ fg_status fg_sim_get_state(fg_sim *sim, double *dens, double *vrad, double *vthe
 fg_status fg_sim_set_state(fg_sim *sim, const double *dens, const double *vrad, const double *vtheta,
                            const double *energy);
 /* scalars: "PhysicalTime", "OmegaFrame", "LostMass", "dt", "dtemp",
- * "MassTaper", "DT", "NTOT", "NINTERM", "NRAD", "NSEC", "last_cfl" */
+ * "MassTaper", "DT", "NTOT", "NINTERM", "NRAD", "NSEC", "last_cfl",
+ * "mdcp", "mdcp1", "exces_mdcp" (diagnostics) */
 fg_status fg_sim_get(fg_sim *sim, const char *name, double *value);
 fg_status fg_sim_nplanets(fg_sim *sim, int *nb);
 /* out = {x, y, vx, vy, mass} */

@@ -134,11 +134,12 @@ struct or_sim {
 
   /* ---- scalars ---- */
   double PhysicalTime, PhysicalTimeInitial, OmegaFrame, MassTaper;
-  double LostMass, mdcp, exces_mdcp, HillRadius;
+  double LostMass, mdcp, mdcp1, exces_mdcp, HillRadius;
+  int Diagnostics; /* the commented-out Main.cu:302,314-315 / SourceEuler.cu:297-298 calls */
   pair_t DiskOnPrimaryAcceleration, IndirectTerm;
   double Xplanet_ps, Yplanet_ps; /* Psys.cu:4 statics */
   int init_counter;              /* SourceEuler.cu:58 */
-  double globalforce[44];        /* Force.cu:306, dimfxy = 11 */
+  double globalforce[4 * 64];    /* Force.cu:93, dimfxy <= 64 (the reference uses 11) */
   double fx_inner, fy_inner, fx_ex_inner, fy_ex_inner;
   double fx_outer, fy_outer, fx_ex_outer, fy_ex_outer;
   int dimfxy;
@@ -1014,6 +1015,94 @@ void or_compute_accel(or_sim *s, double x, double y, double rsmoothing, double m
   } else {
     out2[0] = s->fx_inner + s->fx_outer;
     out2[1] = s->fy_inner + s->fy_outer;
+  }
+}
+
+/* Force.cu:102-135 for any dimfxy (1..64): the GlobalForce array */
+void or_compute_force_hill(or_sim *s, double x, double y, double rsmoothing, int dimfxy, double a, double rh,
+                           double *globalforce) {
+  if (dimfxy < 1 || dimfxy > 64) return;
+  compute_force(s, x, y, rsmoothing, 0.0, dimfxy, a, rh);
+  memcpy(globalforce, s->globalforce, sizeof(double) * 4 * dimfxy);
+}
+
+/* CircumPlanetaryMassKernel (Kernels.cu:236-251) + DeviceReduce, as
+ * CircumPlanetaryMass (SourceEuler.cu:549-563) with a given centre and
+ * radius */
+double or_circumplanetary_mass_at(or_sim *s, double xpl, double ypl, double hill) {
+  const int nr = s->nrad, ns = s->nsec;
+  const long n = (long)nr * ns;
+  double *m = dalloc(n);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < ns; j++) {
+      long l = (long)i * ns + j;
+      double dist = sqrt((s->CellAbscissa[l] - xpl) * (s->CellAbscissa[l] - xpl) +
+                         (s->CellOrdinate[l] - ypl) * (s->CellOrdinate[l] - ypl));
+      m[l] = dist < hill ? s->Surf[i] * s->Dens[l] : 0.0;
+    }
+  double r = or_device_reduce(m, (int)n);
+  free(m);
+  return r;
+}
+
+/* SourceEuler.cu:549-563: planet 0, HillRadius of Psys.cu:140 */
+double or_circumplanetary_mass(or_sim *s) {
+  if (s->nb < 1) return 0.0;
+  return or_circumplanetary_mass_at(s, s->px[0], s->py[0], s->HillRadius);
+}
+
+/* Main.cu:302 (commented out in the reference) */
+void or_enable_diagnostics(or_sim *s) {
+  s->Diagnostics = 1;
+  s->mdcp = or_circumplanetary_mass(s);
+}
+
+/* UpdateLog, Force.cu:19-84 (dimfxy = 11, Main.cu:74) */
+void or_update_log(or_sim *s, int TimeStep) {
+  const int dimfxy = 11;
+  for (int i = 0; i < s->nb; i++) {
+    double x = s->px[i], y = s->py[i], vx = s->pvx[i], vy = s->pvy[i], m = s->pmass[i];
+    double r = sqrt(x * x + y * y);
+    double a = sqrt(x * x + y * y);
+    double rh = pow(m / 3., 1. / 3.) * a + 1e-15;
+    double smoothing = s->RocheSmoothing ? r * pow(m / 3., 1. / 3.) * s->ROCHESMOOTHING : compute_smoothing(s, r);
+    compute_force(s, x, y, smoothing, m, dimfxy, a, rh);
+    const double *gf = s->globalforce;
+    char path[1400];
+    snprintf(path, sizeof path, "%stqwk%d.dat", s->OUTPUTDIR, i);
+    FILE *f = fopen(path, "a");
+    if (!f) continue;
+    fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep,
+            x * s->fy_inner - y * s->fx_inner, x * s->fy_outer - y * s->fx_outer,
+            x * s->fy_ex_inner - y * s->fx_ex_inner, x * s->fy_ex_outer - y * s->fx_ex_outer,
+            vx * s->fx_inner + vy * s->fy_inner, vx * s->fx_outer + vy * s->fy_outer,
+            vx * s->fx_ex_inner + vy * s->fy_ex_inner, vx * s->fx_ex_outer + vy * s->fy_ex_outer, s->PhysicalTime);
+    fclose(f);
+    if (s->SelfGravity) continue;
+    for (int k = 0; k < dimfxy; k++) {
+      snprintf(path, sizeof path, "%stqwk%d_%d.dat", s->OUTPUTDIR, i, k);
+      f = fopen(path, "a");
+      if (!f) continue;
+      fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep, x * gf[2 * dimfxy + k] - y * gf[k],
+              x * gf[3 * dimfxy + k] - y * gf[dimfxy + k], vx * gf[k] + vy * gf[2 * dimfxy + k],
+              vx * gf[dimfxy + k] + vy * gf[3 * dimfxy + k], s->PhysicalTime);
+      fclose(f);
+    }
+  }
+}
+
+/* WriteBigPlanetSystemFile, Output.cu:62-107 */
+void or_write_bigplanet(or_sim *s, int TimeStep) {
+  for (int k = 0; k < s->nb; k++) {
+    char path[1400];
+    snprintf(path, sizeof path, "%sbigplanet%d.dat", s->OUTPUTDIR, k);
+    FILE *f = fopen(path, "a");
+    if (!f) continue;
+    fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", TimeStep, s->px[k],
+            s->py[k], s->pvx[k], s->pvy[k], s->pmass[k], s->LostMass, s->PhysicalTime, s->OmegaFrame, s->mdcp1,
+            s->exces_mdcp);
+    fclose(f);
   }
 }
 
@@ -1931,6 +2020,10 @@ int or_substep(or_sim *s) {
     or_transport(s, dt);
     or_apply_boundary(s, dt);
     or_temperature(s);
+    if (s->Diagnostics) { /* SourceEuler.cu:297-298 */
+      s->mdcp1 = or_circumplanetary_mass(s);
+      s->exces_mdcp = s->mdcp1 - s->mdcp;
+    }
   }
   s->init_counter++;
   s->PhysicalTime += dt;
@@ -1986,6 +2079,11 @@ void or_write_outputs(or_sim *s, int idx) {
 
 void or_run(or_sim *s, int it0, int it1) {
   for (int i = it0; i <= it1; i++) {
+    if (s->write_files && s->Diagnostics) { /* Main.cu:312-316: TimeStep of iteration i-1 */
+      int prev = i > 0 ? (i - 1) / s->NINTERM : 0;
+      or_write_bigplanet(s, prev);
+      or_update_log(s, prev);
+    }
     int TimeStep = i / s->NINTERM;
     if (s->write_files && s->NINTERM * TimeStep == i) or_write_outputs(s, TimeStep);
     or_algogas(s);
@@ -2265,7 +2363,8 @@ static const scalar_desc SCALARS[] = {
     SD(CAVITYRATIO),  SD(CAVITYWIDTH),         SD(TRANSITIONRADIUS), SD(TRANSITIONRATIO),
     SD(TRANSITIONWIDTH), SD(LAMBDADOUBLING),   SD(RELEASEDATE), SD(RELEASERADIUS), SD(HillRadius),
     SD(SGP_eps),      SD(SGP_rstep),           SD(SGP_tstep),  SD(COOLINGBETA), SD(ScalingFactor),
-    SD(fx_inner),     SD(fx_outer),            SD(fy_inner),   SD(fy_outer),
+    SD(fx_inner),     SD(fx_outer),            SD(fy_inner),   SD(fy_outer),   SD(mdcp),
+    SD(mdcp1),        SD(exces_mdcp),
 };
 
 double or_get(or_sim *s, const char *name) {

@@ -60,6 +60,17 @@ int or_condition_cfl(or_sim *s, double DeltaT);              /* SourceEuler.cu:5
 void or_compute_accel(or_sim *s, double x, double y, double rsmoothing,
                       double mass, double *out2);            /* SideEuler.cu:135 */
 double or_device_reduce(const double *in, int n);            /* Kernels.cu:364 */
+/* ComputeForce, Force.cu:102-135: 4*dimfxy sums, GlobalForce layout */
+void or_compute_force_hill(or_sim *s, double x, double y, double rsmoothing, int dimfxy,
+                           double a, double rh, double *globalforce);
+double or_circumplanetary_mass_at(or_sim *s, double xpl, double ypl,
+                                  double hill);              /* Kernels.cu:236 */
+double or_circumplanetary_mass(or_sim *s);                   /* SourceEuler.cu:549 */
+/* turn on the diagnostics the reference ships commented out (Main.cu:302,
+ * 314-315; SourceEuler.cu:297-298); call right after or_create */
+void or_enable_diagnostics(or_sim *s);
+void or_update_log(or_sim *s, int TimeStep);                 /* Force.cu:19 */
+void or_write_bigplanet(or_sim *s, int TimeStep);            /* Output.cu:62 */
 void or_fill_forces_arrays(or_sim *s);                       /* Pframeforce.cu:56 */
 void or_apply_boundary(or_sim *s, double dt);                /* SideEuler.cu:31 */
 void or_correct_vtheta(or_sim *s, double domega);            /* SideEuler.cu:181 */

@@ -48,6 +48,15 @@ def oracle_lib(omp=False):
         L.or_device_reduce.argtypes = [_pd, ctypes.c_int]
         L.or_condition_cfl.argtypes = [_vp, ctypes.c_double]
         L.or_compute_accel.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _pd]
+        L.or_compute_force_hill.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, _pd]
+        L.or_circumplanetary_mass_at.restype = ctypes.c_double
+        L.or_circumplanetary_mass_at.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.or_circumplanetary_mass.restype = ctypes.c_double
+        L.or_circumplanetary_mass.argtypes = [_vp]
+        L.or_enable_diagnostics.argtypes = [_vp]
+        L.or_update_log.argtypes = [_vp, ctypes.c_int]
+        L.or_write_bigplanet.argtypes = [_vp, ctypes.c_int]
         for fn in ("or_fill_forces_arrays", "or_sound_speed", "or_pressure", "or_temperature",
                    "or_actualise_velocities", "or_make1d_cs"):
             getattr(L, fn).argtypes = [_vp]
@@ -120,6 +129,20 @@ class Oracle:
 
     def run(self, it0, it1):
         self.L.or_run(self.h, it0, it1)
+
+    def compute_accel(self, x, y, rs):
+        out = np.empty(2)
+        self.L.or_compute_accel(self.h, x, y, rs, 0.0, out.ctypes.data_as(_pd))
+        return out
+
+    def force_hill(self, x, y, rs, dimfxy, a, rh):
+        """ComputeForce (Force.cu:102-135): GlobalForce, 4*dimfxy sums."""
+        out = np.empty(4 * dimfxy)
+        self.L.or_compute_force_hill(self.h, x, y, rs, dimfxy, a, rh, out.ctypes.data_as(_pd))
+        return out
+
+    def circumplanetary_mass(self, xpl, ypl, hill):
+        return self.L.or_circumplanetary_mass_at(self.h, xpl, ypl, hill)
 
     def __getattr__(self, name):
         fn = getattr(self.__dict__.get("L") or oracle_lib(), "or_" + name)
