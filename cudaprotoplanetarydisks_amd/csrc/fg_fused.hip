@@ -62,9 +62,10 @@ struct src_args {
 };
 
 // potential at one cell (FillForcesArraysKernel, Kernels.cu:1234-1252)
-__device__ __forceinline__ double cell_potential(const fg_planets_arg &P, double Rm, double cosp, double sinp,
-                                                 int indirect) {
-  const double InvDistance = 1.0 / Rm;
+// InvDistance = 1.0 / Rm is passed in: the ring table holds the same
+// correctly rounded quotient
+__device__ __forceinline__ double cell_potential(const fg_planets_arg &P, double Rm, double InvDistance, double cosp,
+                                                 double sinp, int indirect) {
   const double x = Rm * cosp, y = Rm * sinp;
   double acc = 0.0;
   for (int k = 0; k < P.nb; k++) {
@@ -93,15 +94,18 @@ __device__ __forceinline__ void ev_consts(const fg_dev_tables &t, const double *
   vrad0 = -3.0 * nu[i] / Rm * t.vrad0_slope;
 }
 
+template <int KTH, int KTW>
 __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
-  __shared__ double sS[HR][HC];   // Sigma after the first damping
-  __shared__ double sA[HR][HC];   // pressure, then VradInt
-  __shared__ double sB[HR][HC];   // potential, then VthetaInt
-  __shared__ double sC[HR][HC];   // vtheta, then TAURR, then Qr
-  __shared__ double sD[HR][HC];   // vrad, then TAUPP, then Qtheta
-  __shared__ double sE[HR][HC];   // TAURP
+  constexpr int KHR = KTH + HALO_R + HALO_B, KHC = KTW + 2 * HALO_C;
+  constexpr int KPT = (KHR * KHC + SRC_TPB - 1) / SRC_TPB;
+  __shared__ double sS[KHR][KHC];   // Sigma after the first damping
+  __shared__ double sA[KHR][KHC];   // pressure, then VradInt
+  __shared__ double sB[KHR][KHC];   // potential, then VthetaInt
+  __shared__ double sC[KHR][KHC];   // vtheta, then TAURR, then Qr
+  __shared__ double sD[KHR][KHC];   // vrad, then TAUPP, then Qtheta
+  __shared__ double sE[KHR][KHC];   // TAURP
   const int nr = a.nr, ns = a.ns;
-  const int i0 = blockIdx.y * TH, j0 = blockIdx.x * TW;
+  const int i0 = blockIdx.y * KTH, j0 = blockIdx.x * KTW;
   const int tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
   const double dt = a.dt;
@@ -110,15 +114,15 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
 
   // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
   // all of a thread's loads are issued before any of them is used
-  double ls[SRC_PER_THREAD], lvr[SRC_PER_THREAD], lvt[SRC_PER_THREAD];
-  int lj[SRC_PER_THREAD];
+  double ls[KPT], lvr[KPT], lvt[KPT];
+  int lj[KPT];
 #pragma unroll
-  for (int k = 0; k < SRC_PER_THREAD; k++) {
+  for (int k = 0; k < KPT; k++) {
     const int q = tid + k * SRC_TPB;
     ls[k] = lvr[k] = lvt[k] = 0.0;
     lj[k] = 0;
-    if (q < HR * HC) {
-      const int r = q / HC, cc = q - r * HC, i = GI(r);
+    if (q < KHR * KHC) {
+      const int r = q / KHC, cc = q - r * KHC, i = GI(r);
       if (i >= 0 && i <= nr) {
         const int j = GJ(cc);
         const long l = (long)i * ns + j;
@@ -129,11 +133,21 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       }
     }
   }
+  // damping constants of the tile's rings, once per ring while the loads
+  // above are in flight (Kernels.cu:978-1004)
+  __shared__ double evc[KHR][3];
+  if (a.evanescent) {
+    for (int r = tid; r < KHR; r += SRC_TPB) {
+      const int i = GI(r);
+      if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
+    }
+    __syncthreads();
+  }
 #pragma unroll
-  for (int k = 0; k < SRC_PER_THREAD; k++) {
+  for (int k = 0; k < KPT; k++) {
     const int q = tid + k * SRC_TPB;
-    if (q >= HR * HC) break;
-    int r = q / HC, cc = q - r * HC;
+    if (q >= KHR * KHC) break;
+    int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
     double s = 0.0, vr = 0.0, vt = 0.0, p = 0.0, phi = 0.0;
     if (i >= 0 && i <= nr) {
@@ -144,15 +158,13 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       if (i < nr) {
         if (a.corotating) vt = vt - a.domega * t.Rmed[i];
         if (a.evanescent && t.ev_zone[i]) {
-          double lambda, vrad0, vtheta0;
-          ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
-          vr = damp_toward(vr, lambda, vrad0);
-          vt = damp_toward(vt, lambda, vtheta0);
-          s = damp_toward(s, lambda, t.SigmaMed[i]);
+          vr = damp_toward(vr, evc[r][0], evc[r][1]);
+          vt = damp_toward(vt, evc[r][0], evc[r][2]);
+          s = damp_toward(s, evc[r][0], t.SigmaMed[i]);
         }
         const double cs = t.cs_iso[i];
         p = s * cs * cs;
-        phi = cell_potential(a.P, t.Rmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
+        phi = cell_potential(a.P, t.Rmed[i], t.invRmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
       }
     }
     sS[r][cc] = s;
@@ -163,15 +175,15 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   }
   __syncthreads();
 
-  // ---- S1: Substep1 (Kernels.cu:13-53) on rows [1, HR), cols [1, HC-1) ----
-  double regA[SRC_PER_THREAD], regB[SRC_PER_THREAD];
+  // ---- S1: Substep1 (Kernels.cu:13-53) on rows [1, KHR), cols [1, KHC-1) ----
+  double regA[KPT], regB[KPT];
   {
     int k = 0;
-    for (int q = tid; q < HR * HC; q += SRC_TPB, k++) {
-      int r = q / HC, cc = q - r * HC;
+    for (int q = tid; q < KHR * KHC; q += SRC_TPB, k++) {
+      int r = q / KHC, cc = q - r * KHC;
       int i = GI(r);
       double vri = 0.0, vti = 0.0;
-      if (r >= 1 && cc >= 1 && cc < HC - 1 && i >= 0 && i < nr) {
+      if (r >= 1 && cc >= 1 && cc < KHC - 1 && i >= 0 && i < nr) {
         if (i > 0) {
           double gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0 / (sS[r][cc] + sS[r - 1][cc]) * t.invdiffRmed[i];
           double gradphi = (sB[r][cc] - sB[r - 1][cc]) * t.invdiffRmed[i];
@@ -193,29 +205,29 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       regB[k] = vti;
     }
     if (a.opress) {  // pressure for the temperature output, own cells only
-      for (int q = tid; q < TH * TW; q += SRC_TPB) {
-        int r = q / TW + HALO_R, cc = q % TW + HALO_C;
+      for (int q = tid; q < KTH * KTW; q += SRC_TPB) {
+        int r = q / KTW + HALO_R, cc = q % KTW + HALO_C;
         int i = GI(r), j = j0 + cc - HALO_C;
         if (i < nr && j < ns) a.opress[(long)i * ns + j] = sA[r][cc];
       }
     }
     __syncthreads();
     k = 0;
-    for (int q = tid; q < HR * HC; q += SRC_TPB, k++) {
-      int r = q / HC, cc = q - r * HC;
+    for (int q = tid; q < KHR * KHC; q += SRC_TPB, k++) {
+      int r = q / KHC, cc = q - r * KHC;
       sA[r][cc] = regA[k];  // VradInt (0 on rows 0 and NRAD, as never written)
       sB[r][cc] = regB[k];  // VthetaInt
     }
   }
   __syncthreads();
 
-  // ---- S2: viscous stress (Kernels.cu:733-771) on rows [1, HR-1), cols [2, HC-2) ----
+  // ---- S2: viscous stress (Kernels.cu:733-771) on rows [1, KHR-1), cols [2, KHC-2) ----
   const double invdphi = t.invdphi;
   const double onethird = 1.0 / 3.0;
-  for (int q = tid; q < HR * HC; q += SRC_TPB) {
-    int r = q / HC, cc = q - r * HC;
+  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+    int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
-    if (r < 1 || r >= HR - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
+    if (r < 1 || r >= KHR - 1 || cc < 2 || cc >= KHC - 2 || i < 0 || i >= nr) continue;
     const double *VrR = &sA[r][0];
     double drr = (sA[r + 1][cc] - VrR[cc]) * t.invdiffRsup[i];
     double dpp = (sB[r][cc + 1] - sB[r][cc]) * invdphi * t.invRmed[i] + 0.5 * (sA[r + 1][cc] + VrR[cc]) * t.invRmed[i];
@@ -233,11 +245,11 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __syncthreads();
 
   // ---- S3: viscous velocity update (Kernels.cu:118-150), in place ----
-  for (int q = tid; q < HR * HC; q += SRC_TPB) {
-    int r = q / HC, cc = q - r * HC;
+  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+    int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
-    if (r < 2 || r >= HR - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
-    if (i < nr - 1 && r < HR - 2)
+    if (r < 2 || r >= KHR - 1 || cc < 3 || cc >= KHC - 3 || i < 1 || i >= nr) continue;
+    if (i < nr - 1 && r < KHR - 2)
       sB[r][cc] += dt * t.invRmed[i] *
                    ((t.Rsup[i] * sE[r + 1][cc] - t.Rinf[i] * sE[r][cc]) * t.invdiffRsup[i] +
                     (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5 * (sE[r][cc] + sE[r + 1][cc])) /
@@ -249,19 +261,19 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   }
   __syncthreads();
 
-  // ---- S4: artificial pressure (Kernels.cu:409-434) on rows [3, HR-1), cols [3, HC-4) ----
-  for (int q = tid; q < HR * HC; q += SRC_TPB) {
-    int r = q / HC, cc = q - r * HC;
+  // ---- S4: artificial pressure (Kernels.cu:409-434) on rows [3, KHR-1), cols [3, KHC-4) ----
+  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+    int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
-    if (r < 3 || r >= HR - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
+    if (r < 3 || r >= KHR - 1 || cc < 3 || cc >= KHC - 4 || i < 0 || i >= nr) continue;
     sC[r][cc] = art_visc_q(sS[r][cc], sA[r + 1][cc] - sA[r][cc]);
     sD[r][cc] = art_visc_q(sS[r][cc], sB[r][cc + 1] - sB[r][cc]);
   }
   __syncthreads();
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2, store ----
-  for (int q = tid; q < TH * TW; q += SRC_TPB) {
-    int r = q / TW + HALO_R, cc = q % TW + HALO_C;
+  for (int q = tid; q < KTH * KTW; q += SRC_TPB) {
+    int r = q / KTW + HALO_R, cc = q % KTW + HALO_C;
     int i = GI(r), j = j0 + cc - HALO_C;
     if (i > nr || j >= ns) continue;
     long l = (long)i * ns + j;
@@ -283,11 +295,9 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     double vtnew = sB[r][cc] - dt * 2.0 / dens2 * tempint * invdxtheta;
     double s = sS[r][cc];
     if (a.evanescent && t.ev_zone[i]) {
-      double lambda, vrad0, vtheta0;
-      ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
-      vrnew = damp_toward(vrnew, lambda, vrad0);
-      vtnew = damp_toward(vtnew, lambda, vtheta0);
-      s = damp_toward(s, lambda, t.SigmaMed[i]);
+      vrnew = damp_toward(vrnew, evc[r][0], evc[r][1]);
+      vtnew = damp_toward(vtnew, evc[r][0], evc[r][2]);
+      s = damp_toward(s, evc[r][0], t.SigmaMed[i]);
     }
     a.odens[l] = s;
     a.ovr[l] = vrnew;
@@ -660,6 +670,36 @@ __global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr
   force_rings_final(fpart, nr, out2);
 }
 
+// Left-to-right sum of ring[0..n) in one lane, the order of the
+// reference's one-thread ring loops (Kernels.cu:1276-1279).  Values come
+// from LDS in blocks of 16, the next block's loads issued before the
+// current block's adds, so the LDS latency overlaps the dependent adds.
+__device__ __forceinline__ double seq_sum_lds(const double *ring, int n) {
+  double m = 0.0;
+  int j = 0;
+  if (n >= 32) {
+    double a[16], b[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) a[q] = ring[q];
+    for (; j + 32 <= n; j += 32) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) b[q] = ring[j + 16 + q];
+      __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the adds
+#pragma unroll
+      for (int q = 0; q < 16; q++) m += a[q];
+      // the next a block, or (past the last full pair) a harmless re-read
+      const int jn = (j + 48 <= n) ? j + 32 : j;
+#pragma unroll
+      for (int q = 0; q < 16; q++) a[q] = ring[jn + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; q++) m += b[q];
+    }
+  }
+  for (; j < n; j++) m += ring[j];
+  return m;
+}
+
 // one ring; ring holds NSEC doubles (the ring's final vtheta)
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[256];
@@ -726,8 +766,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   }
   __syncthreads();
   if (tid == 0) {
-    double m = 0.0;
-    for (int j = 0; j < ns; j++) m += ring[j];
+    double m = seq_sum_lds(ring, ns);
     m /= (double)ns;
     a.vmoy[i] = m;
     s_vmoy = m;
@@ -768,9 +807,16 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   if (tid == 0) a.newdt[i] = red[0];
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs.  Each XCD gets a
+// contiguous band of rings, so neighbouring rings, which read each other's
+// momenta and densities, are processed side by side under one L2.
+#define FG_NXCD 8
 __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   extern __shared__ double lds_fin[];
-  fin_ring(a, blockIdx.x, lds_fin);
+  const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
+  const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
+  if (i >= a.nr) return;
+  fin_ring(a, i, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -825,17 +871,17 @@ __device__ __forceinline__ float cell_potential_planets_f(const fg_planets_arg &
   return acc;
 }
 
-// CorrectVtheta and the first damping pass of one cell's state (FP64)
-__device__ __forceinline__ void src32_adjust(const src32_args &a, int i, double &s, double &vr, double &vt) {
+// CorrectVtheta and the first damping pass of one cell's state (FP64);
+// ev = the ring's damping constants (lambda, vrad0, vtheta0)
+__device__ __forceinline__ void src32_adjust(const src32_args &a, int i, const double *ev, double &s, double &vr,
+                                             double &vt) {
   const fg_dev_tables &t = a.t;
   if (i < a.nr) {
     if (a.corotating) vt = vt - a.domega * t.Rmed[i];
     if (a.evanescent && t.ev_zone[i]) {
-      double lambda, vrad0, vtheta0;
-      ev_consts(t, a.nu, i, a.dt, a.omega_frame, lambda, vrad0, vtheta0);
-      vr = damp_toward(vr, lambda, vrad0);
-      vt = damp_toward(vt, lambda, vtheta0);
-      s = damp_toward(s, lambda, t.SigmaMed[i]);
+      vr = damp_toward(vr, ev[0], ev[1]);
+      vt = damp_toward(vt, ev[0], ev[2]);
+      s = damp_toward(s, ev[0], t.SigmaMed[i]);
     }
   }
 }
@@ -852,6 +898,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   float(*sE)[HC] = (float(*)[HC])(lds + 5 * hr * HC);  // TAURP
   float(*incR)[TW] = (float(*)[TW])(lds + 6 * hr * HC);  // own-cell velocity increments
   float(*incT)[TW] = (float(*)[TW])(lds + 6 * hr * HC + th * TW);
+  __shared__ double evc[SRC32_MAX_TH + HALO_R + HALO_B][3];  // damping constants per tile row
   const int nr = a.nr, ns = a.ns;
   const int i0 = by * th, j0 = bx * TW;
   const int tid = threadIdx.x;
@@ -880,6 +927,15 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
         }
       }
     }
+  // damping constants of the tile's rings, once per ring while the loads
+  // above are in flight (Kernels.cu:978-1004)
+  if (a.evanescent) {
+    for (int r = tid; r < hr; r += SRC_TPB) {
+      const int i = GI(r);
+      if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int k = 0; k < SRC32_PER_THREAD; k++) {
     const int q = tid + k * SRC_TPB;
@@ -890,7 +946,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     if (i >= 0 && i <= nr) {
       int j = GJ(cc);
       double sd = ls[k], vrd = lvr[k], vtd = lvt[k];
-      src32_adjust(a, i, sd, vrd, vtd);
+      src32_adjust(a, i, evc[r], sd, vrd, vtd);
       s = (float)sd;
       vr = (float)vrd;
       vt = (float)(vtd - VREF(i));
@@ -1048,7 +1104,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     if (i > nr || j >= ns) continue;
     long l = (long)i * ns + j;
     double s = os[k], vr = ovr[k], vt = ovt[k];
-    src32_adjust(a, i, s, vr, vt);
+    src32_adjust(a, i, evc[r], s, vr, vt);
     if (a.opress && i < nr) a.opress[l] = s * t.cs_iso[i] * t.cs_iso[i];
     if (i == nr) {  // ghost ring
       a.odens[l] = s;
@@ -1064,11 +1120,9 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     float dq = (sD[r][cc] - sD[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * (float)t.invdxtheta[i];
     double vtnew = vt + (double)(incT[ro][co] - dtf * dq);
     if (a.evanescent && t.ev_zone[i]) {
-      double lambda, vrad0, vtheta0;
-      ev_consts(t, a.nu, i, dt, a.omega_frame, lambda, vrad0, vtheta0);
-      vrnew = damp_toward(vrnew, lambda, vrad0);
-      vtnew = damp_toward(vtnew, lambda, vtheta0);
-      s = damp_toward(s, lambda, t.SigmaMed[i]);
+      vrnew = damp_toward(vrnew, evc[r][0], evc[r][1]);
+      vtnew = damp_toward(vtnew, evc[r][0], evc[r][2]);
+      s = damp_toward(s, evc[r][0], t.SigmaMed[i]);
     }
     a.odens[l] = s;
     a.ovr[l] = vrnew;
@@ -1564,7 +1618,8 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC_TPB),
                        src32_lds_bytes(th), c->stream, s3, th);
   } else {
-    hipLaunchKernelGGL(k_src, dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream, sa);
+    hipLaunchKernelGGL((k_src<TH, TW>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream,
+                       sa);
   }
   FG_LAUNCH_CHECK();
   // k_trad
@@ -1649,7 +1704,8 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.fast = c->p.fast_transport;
   fa.fpart = fp32 ? c->fu_fpart : nullptr;
   prof_mark(c);
-  hipLaunchKernelGGL(k_fin, dim3(nr), dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
+  hipLaunchKernelGGL(k_fin, dim3(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD)), dim3(256), (size_t)ns * sizeof(double),
+                     c->stream, fa);
   FG_LAUNCH_CHECK();
   prof_mark(c);
   return fused_reductions(c, DeltaT_next, fp32);
@@ -1699,7 +1755,9 @@ __device__ __forceinline__ int2 uniform2(int2 w) {
   return make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y));
 }
 
-__global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
+// two workgroups per CU (at most 256 VGPRs): the per-disk arguments fill
+// the SGPRs and spill into VGPR lanes, which would otherwise cross 256
+__global__ __launch_bounds__(SRC_TPB, 2) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
   extern __shared__ float lds_bsrc[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int4 w = uniform4(items[it]);
