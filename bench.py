@@ -7,9 +7,11 @@ physics (isothermal h = 0.05, nu = 1e-5, one q = 1e-3 planet, evanescent
 edges) and a seeded 1e-4 white-noise perturbation of Sigma.  Inputs are
 resident in HBM when the timed region starts.
 
-Prints one JSON line: zone-updates/s of the GPU path, the roofline of the
-dominant kernel (HIP-event timing inside the library) and the oracle's
-multicore FP64 rate on a bounded sample of the same workload.
+Prints one JSON line: zone-updates/s of the GPU path (FP64 by default),
+the roofline of the dominant kernel (HIP-event timing inside the library),
+the oracle's multicore FP64 rate on a bounded sample of the same workload,
+the same substep in the other precision ("other_precision": the FP32 bulk
+mode beside the FP64 headline) and a batched ensemble of drawn disks.
 """
 import argparse
 import ctypes
@@ -97,9 +99,64 @@ def ensemble_leg(disks, steps, warmup, seed=SEED):
         return {"disks": disks, "seed": seed, "grids_nrad": grids, "zones": int(sum(zones)),
                 "batch_substeps": steps, "ms_per_batch_substep": round(1e3 * wall / steps, 4),
                 "value": round(zu / wall, 1), "unit": "zone-updates/s", "dtype": "f32",
-                "launch": "one captured hipGraph per batch substep, one persistent launch per phase"}
+                "launch": "one captured hipGraph per batch substep, one launch per phase over the (disk, tile) work list"}
     finally:
         ens.close()
+
+
+def gpu_leg(deck, precision, path, steps, warmup):
+    """Time `steps` substeps after `warmup` (inputs resident in HBM), then a
+    separate pass with per-kernel HIP events.  Returns (zone-updates/s,
+    seconds, {kernel: ms per launch}, all-finite)."""
+    import numpy as np
+    from cudaprotoplanetarydisks_amd import Simulation, lib
+    from cudaprotoplanetarydisks_amd.native import call
+    sim = Simulation(deck, fused=(path == "fused"), precision=precision, noise_seed=SEED, noise_amp=1e-4)
+    try:
+        ctx = sim.ctx.h
+        for _ in range(warmup):
+            sim.substep()
+        call("fg_sync", ctx)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.substep()
+        call("fg_sync", ctx)
+        elapsed = time.perf_counter() - t0
+        # kernel timing (separate pass so events do not perturb the clock)
+        prof = {}
+        if path == "fused":
+            call("fg_profile", ctx, 1)
+            for _ in range(min(steps, 50)):
+                sim.substep()
+            ms = (ctypes.c_double * 6)()
+            calls = (ctypes.c_long * 6)()
+            n = lib().fg_profile_read(ctx, ms, calls, 6)
+            call("fg_profile", ctx, 0)
+            prof = {KERNELS[k]: ms[k] / max(1, calls[k]) for k in range(n)}
+        d, _, _ = sim.state()
+        nr, ns = sim.nrad, sim.nsec
+        return nr * ns * steps / elapsed, elapsed, prof, bool(np.isfinite(d).all())
+    finally:
+        sim.close()
+
+
+def roofline_of(prof, nr, ns, precision):
+    """Roofline entry of the dominant (longest) kernel: algorithmic bytes
+    per launch over its event-timed duration.  Both precisions keep the
+    state and the quantities passed between kernels in FP64."""
+    if not prof:
+        return None
+    dom = max(prof, key=prof.get)
+    ab = algorithmic_bytes(nr, ns, 8)[dom]
+    achieved = ab / (prof[dom] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(f"{precision}:{dom}")
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
 
 
 def main():
@@ -111,8 +168,9 @@ def main():
     ap.add_argument("--nsec", type=int, default=3072)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--path", choices=["fused", "routine"], default="fused")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-alt", action="store_true", help="skip the leg in the other precision")
     ap.add_argument("--ensemble-disks", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=100)
     args = ap.parse_args()
@@ -120,56 +178,19 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(every GPU box here has one GPU)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-    from cudaprotoplanetarydisks_amd import Simulation, lib
-    from cudaprotoplanetarydisks_amd.native import call
-
     nr, ns = args.nrad, args.nsec
     with tempfile.TemporaryDirectory() as tmp:
         deck = write_workload_deck(tmp, nr, ns)
-        sim = Simulation(deck, fused=(args.path == "fused"), precision=args.precision,
-                         noise_seed=SEED, noise_amp=1e-4)
-        ctx = sim.ctx.h
-        for _ in range(args.warmup):
-            sim.substep()
-        call("fg_sync", ctx)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            sim.substep()
-        call("fg_sync", ctx)
-        t1 = time.perf_counter()
-        elapsed = t1 - t0
-        # kernel timing (separate pass so events do not perturb the clock)
-        prof = {}
-        if args.path == "fused":
-            call("fg_profile", ctx, 1)
-            for _ in range(min(args.steps, 50)):
-                sim.substep()
-            ms = (ctypes.c_double * 6)()
-            calls = (ctypes.c_long * 6)()
-            n = lib().fg_profile_read(ctx, ms, calls, 6)
-            call("fg_profile", ctx, 0)
-            prof = {KERNELS[k]: ms[k] / max(1, calls[k]) for k in range(n)}
-        d, _, _ = sim.state()
-        finite = bool(np.isfinite(d).all())
-        sim.close()
-        # both precisions keep the state (Sigma, vrad, vtheta) and the
-        # quantities passed between kernels in FP64; FP32 is the arithmetic
-        word = 8
-        zus = nr * ns * args.steps / elapsed
-        b_alg = 48  # read + write of Sigma, vrad, vtheta per zone-update
-        roofline = None
-        if prof:
-            dom = max(prof, key=prof.get)
-            ab = algorithmic_bytes(nr, ns, word)[dom]
-            achieved = ab / (prof[dom] * 1e-3) / 1e9
-            traffic = None
-            tfile = os.path.join(ROOT, "profiles", "traffic.json")
-            if os.path.exists(tfile):
-                traffic = json.load(open(tfile)).get(f"{args.precision}:{dom}")
-            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                        "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+        zus, elapsed, prof, finite = gpu_leg(deck, args.precision, args.path, args.steps, args.warmup)
+        roofline = roofline_of(prof, nr, ns, args.precision)
+        # the other precision on the same workload, reported beside the headline
+        alt = None
+        if not args.no_alt and args.path == "fused":
+            other = "fp32" if args.precision == "fp64" else "fp64"
+            a_zus, a_el, a_prof, a_fin = gpu_leg(deck, other, "fused", args.steps, args.warmup)
+            alt = {"precision": other, "value": round(a_zus, 1), "unit": "zone-updates/s",
+                   "ms_per_step": round(1e3 * a_el / args.steps, 4), "finite": a_fin,
+                   "roofline": roofline_of(a_prof, nr, ns, other)}
         ens = None
         if args.ensemble_disks > 0:
             ens = ensemble_leg(args.ensemble_disks, args.ensemble_steps, 10)
@@ -181,6 +202,7 @@ def main():
             cpu = {"value": round(rate, 1), "unit": "zone-updates/s", "cores": threads, "kind": "port",
                    "sample": f"oracle FP64 (OpenMP), {nr}x{ns}, 1 warm-up + {args.cpu_steps} substeps "
                              f"({secs:.1f} s)"}
+    b_alg = 48  # read + write of Sigma, vrad, vtheta (FP64) per zone-update
     out = {
         "metric": METRIC, "value": round(zus, 1), "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
@@ -194,7 +216,10 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
         "finite": finite,
+        "other_precision": alt,
     }
+    if alt and cpu:
+        alt["gpu_vs_cpu"] = round(alt["value"] / cpu["value"], 1)
     print(json.dumps(out))
 
 
