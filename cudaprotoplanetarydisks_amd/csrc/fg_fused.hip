@@ -461,10 +461,10 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
 
 // ------------------------------------------------------------ k_taz ----
 // OneWindTheta (TransportEuler.cu:177-194) + the vtheta half of
-// ComputeVelocities (Kernels.cu:1206-1207) for one ring per 256-thread
-// workgroup.  Thread t owns sectors j = t + 256 k (k < C); the five
-// quantities stay in registers, neighbours are exchanged through two LDS
-// rings.  VMed comes from the ring-mean waves of k_trad.
+// ComputeVelocities (Kernels.cu:1206-1207).  FP64: k_tazw below, wave
+// chunks.  FP32 (k_taz32 further down): one ring per 256-thread workgroup,
+// thread t owning sectors j = t + 256 k (k < C), neighbours exchanged
+// through LDS rings.  VMed comes from the ring-mean waves of k_trad.
 #define TAZ_TPB 256
 
 struct taz_args {
@@ -478,76 +478,132 @@ struct taz_args {
   int nr, ns, fast;
 };
 
-// cells of thread t: j = t + TAZ_TPB k, k < C (all of them when FULL,
-// i.e. NSEC == TAZ_TPB * C)
-#define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
 
-// QuantitiesAdvection (TransportEuler.cu:229-245) with velocity V (one
-// value per cell, or the ring's uniform residual when UNIFORM).  W, D and
-// F are three NSEC-long LDS rings (cell values, slopes, fluxes): with a
-// separate flux ring each quantity needs three barriers.
-template <int C, bool FULL, bool UNIFORM>
-__device__ __forceinline__ void azimuthal_sweep(double (&q)[NQ][C], const double (&Vc)[C], double Vu, double *W,
-                                                double *D, double *Fl, int ns, int i, const fg_dev_tables &t,
-                                                double dt) {
+// ------------------------------------------------------- k_taz, waves ----
+// No workgroup barrier: one 64-lane wave owns a
+// contiguous chunk of 64*CL cells of a ring, CL consecutive cells per lane
+// in registers.  A cell's neighbours are in the lane's own registers, or for
+// a lane's first and last cells in the adjacent lane, fetched by DPP wave
+// shifts.  The chunk carries TAZW_HL halo cells on the left and TAZW_HR on
+// the right: each sweep's result at a cell depends on the cells up to two
+// away, the vtheta recovery on the cell to the left, and the two lanes at
+// the chunk's ends see zeros from the shifts, so the outermost 4 (right)
+// and 5 (left) results of a chunk are discarded.  Chunks of one ring
+// overlap by those halos; each cell is output by exactly one chunk.
+#define TAZW_HL 5
+#define TAZW_HR 4
+
+// x of lane l-1 (lane 0 reads 0): DPP wave_shr:1 on each dword
+__device__ __forceinline__ double lane_prev(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// x of lane l+1 (lane 63 reads 0): DPP wave_shl:1 on each dword
+__device__ __forceinline__ double lane_next(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// QuantitiesAdvection (TransportEuler.cu:229-245) on a lane's CL cells with
+// velocity V (one value per cell, or the ring's uniform residual when
+// UNIFORM); the reference's operations in its order
+template <int CL, bool UNIFORM>
+__device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&Vc)[CL], double Vu,
+                                           const fg_dev_tables &t, int i, double dt) {
   const double invdxtheta = t.invdxtheta[i], dxtheta = t.dxtheta[i];
   const double dxrad = (t.Rsup[i] - t.Rinf[i]) * dt;
   const double invsurf = t.invSurf[i];
-  double dens[C], DS[C];
-#define V(k) (UNIFORM ? Vu : Vc[k])
+#define V(c) (UNIFORM ? Vu : Vc[c])
+  double dens[CL], DS[CL];
   // DensStar from Sigma (ComputeStarTheta(Dens), TransportEuler.cu:232)
-  FOR_CELLS(k, j) {
-    dens[k] = q[NQ - 1][k];
-    W[j] = dens[k];
+#pragma unroll
+  for (int c = 0; c < CL; c++) dens[c] = q[NQ - 1][c];
+  {
+    const double Wm = lane_prev(dens[CL - 1]), Wp = lane_next(dens[0]);
+    double D[CL];
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      D[c] = vl_theta_slope(c ? dens[c - 1] : Wm, dens[c], c < CL - 1 ? dens[c + 1] : Wp, invdxtheta);
+    const double Dm = lane_prev(D[CL - 1]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      DS[c] = vl_theta_star(V(c), dt, c ? dens[c - 1] : Wm, dens[c], c ? D[c - 1] : Dm, D[c], dxtheta);
   }
-  __syncthreads();
-  FOR_CELLS(k, j) D[j] = vl_theta_slope(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
-  __syncthreads();
-  FOR_CELLS(k, j) {
-    int jm = j == 0 ? ns - 1 : j - 1;
-    DS[k] = vl_theta_star(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta);
-  }
-  __syncthreads();
   // VanLeerTheta of each quantity, Sigma last (TransportEuler.cu:234-243)
 #pragma unroll
   for (int qq = 0; qq < NQ; qq++) {
-    FOR_CELLS(k, j) W[j] = q[qq][k] / (dens[k] + 1e-20);
-    __syncthreads();
-    FOR_CELLS(k, j) D[j] = vl_theta_slope(W[j == 0 ? ns - 1 : j - 1], W[j], W[j + 1 == ns ? 0 : j + 1], invdxtheta);
-    __syncthreads();
-    double F[C];
-    FOR_CELLS(k, j) {
-      int jm = j == 0 ? ns - 1 : j - 1;
-      double QS = vl_theta_star(V(k), dt, W[jm], W[j], D[jm], D[j], dxtheta);
-      F[k] = dxrad * QS * DS[k] * V(k);
-      Fl[j] = F[k];
+    // one quantity at a time: keeps the live registers to this quantity's
+    // stencil (the scheduler would otherwise interleave all five)
+    __builtin_amdgcn_sched_barrier(0);
+    double W[CL], D[CL], F[CL];
+#pragma unroll
+    for (int c = 0; c < CL; c++) W[c] = q[qq][c] / (dens[c] + 1e-20);
+    const double Wm = lane_prev(W[CL - 1]), Wp = lane_next(W[0]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      D[c] = vl_theta_slope(c ? W[c - 1] : Wm, W[c], c < CL - 1 ? W[c + 1] : Wp, invdxtheta);
+    const double Dm = lane_prev(D[CL - 1]);
+#pragma unroll
+    for (int c = 0; c < CL; c++) {
+      const double QS = vl_theta_star(V(c), dt, c ? W[c - 1] : Wm, W[c], c ? D[c - 1] : Dm, D[c], dxtheta);
+      F[c] = dxrad * QS * DS[c] * V(c);
     }
-    __syncthreads();
-    FOR_CELLS(k, j) {
-      double varq = F[k];
-      varq -= Fl[j + 1 == ns ? 0 : j + 1];
-      q[qq][k] += varq * invsurf;
+    const double Fp = lane_next(F[0]);
+#pragma unroll
+    for (int c = 0; c < CL; c++) {
+      double varq = F[c];
+      varq -= (c < CL - 1 ? F[c + 1] : Fp);
+      q[qq][c] += varq * invsurf;
     }
   }
-  __syncthreads();
 #undef V
 }
 
-// lds holds three NSEC-long double rings
-template <int C, bool FULL>
-__global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
-  extern __shared__ double lds[];
-  double *W = lds, *D = lds + a.ns, *Fl = lds + 2 * a.ns;
-  const int ns = a.ns, i = blockIdx.x, tid = threadIdx.x;
+// one wave per (chunk, ring): blockIdx.x = chunk, blockIdx.y = ring
+template <int CL>
+__device__ __forceinline__ void tazw_body(const taz_args &a) {
+  constexpr int N = 64 * CL, P = N - TAZW_HL - TAZW_HR;
+  __shared__ double stage[N];
+  const int ns = a.ns, i = blockIdx.y, l = threadIdx.x;
+  const int g0 = (int)blockIdx.x * P - TAZW_HL;  // unwrapped index of the chunk's first cell
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
+  // loads: coalesced global reads into the chunk's LDS stage, then CL
+  // consecutive cells per lane (a one-wave workgroup: the barriers only
+  // order the LDS accesses of the wave)
+  double q[NQ][CL], vres[CL];
   const double VMed = a.vmed[i];
-  double q[NQ][C], vres[C];
-  FOR_CELLS(k, j) {
-    vres[k] = a.vt[base + j] - VMed;  // ComputeResidualsKernel
+  // ring index of this lane's staged cell m: one modulo, then steps of 64
+  int jl[CL];
+  {
+    int j = ((g0 + l) % ns + ns) % ns;
 #pragma unroll
-    for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
+    for (int m = 0; m < CL; m++) {
+      jl[m] = j;
+      j += 64;
+      while (j >= ns) j -= ns;
+    }
+  }
+#pragma unroll
+  for (int qq = 0; qq <= NQ; qq++) {
+    const double *src = qq < NQ ? a.q[qq] : a.vt;
+#pragma unroll
+    for (int m = 0; m < CL; m++) stage[l + 64 * m] = src[base + jl[m]];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CL; c++) {
+      const double v = stage[l * CL + c];
+      if (qq < NQ)
+        q[qq][c] = v;
+      else
+        vres[c] = v - VMed;  // ComputeResidualsKernel
+    }
+    __syncthreads();
   }
   // ComputeConstantResidualKernel (Kernels.cu:1094-1126)
   const double invdt = 1.0 / a.dt;
@@ -557,37 +613,77 @@ __global__ __launch_bounds__(TAZ_TPB) void k_taz(taz_args a) {
   const int nsh = (int)(long)Nround;
   double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
   if (!a.fast) {
-    FOR_CELLS(k, j) vres[k] += vuni;
+#pragma unroll
+    for (int c = 0; c < CL; c++) vres[c] += vuni;
     vuni = 0.0;
   }
-  if (tid == 0) a.nshift[i] = nsh;
-  azimuthal_sweep<C, FULL, false>(q, vres, 0.0, W, D, Fl, ns, i, t, a.dt);
-  azimuthal_sweep<C, FULL, true>(q, vres, vuni, W, D, Fl, ns, i, t, a.dt);
-  // AdvectSHIFT (Kernels.cu:1173-1190) folded into the stores: the value
-  // of cell j goes to j + Nshift (mod NSEC).  vtheta from the angular
-  // momenta (Kernels.cu:1206-1207) of the shifted ring at j + Nshift uses
-  // the cells j - 1 and j of the unshifted one.
+  if (blockIdx.x == 0 && l == 0) a.nshift[i] = nsh;
+  wave_sweep<CL, false>(q, vres, 0.0, t, i, a.dt);
+  wave_sweep<CL, true>(q, vres, vuni, t, i, a.dt);
+  // vtheta from the angular momenta (Kernels.cu:1206-1207): cells j - 1
+  // and j of the unshifted ring
+  const double Rm = t.Rmed[i];
+  double vt[CL];
+  {
+    const double Wm = lane_prev(q[2][CL - 1]), Sm = lane_prev(q[NQ - 1][CL - 1]);
+#pragma unroll
+    for (int c = 0; c < CL; c++) {
+      const double s = q[NQ - 1][c];
+      vt[c] = ((c ? q[2][c - 1] : Wm) + q[3][c]) / (s + (c ? q[NQ - 1][c - 1] : Sm) + 1e-15) / Rm -
+              Rm * a.omega_frame;
+    }
+  }
+  // stores with AdvectSHIFT (Kernels.cu:1173-1190) folded in: cell j goes
+  // to j + Nshift (mod NSEC); the chunk outputs its cells [HL, HL + P)
+  // that lie in [0, NSEC)
   int sh = nsh % ns;
   if (sh < 0) sh += ns;
-  FOR_CELLS(k, j) {
-    W[j] = q[2][k];
-    D[j] = q[NQ - 1][k];
-  }
-  __syncthreads();
-  const double Rm = t.Rmed[i];
-  FOR_CELLS(k, j) {
-    int jm = j == 0 ? ns - 1 : j - 1;
-    double s = q[NQ - 1][k];
-    double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
-    int jo = j + sh;
-    if (jo >= ns) jo -= ns;
-    a.xd[base + jo] = s;
-    a.xvt[base + jo] = v;
-    a.xrmp[base + jo] = q[0][k];
-    a.xrmm[base + jo] = q[1][k];
+#pragma unroll
+  for (int o = 0; o < 4; o++) {
+    double *dst = o == 0 ? a.xd : o == 1 ? a.xvt : o == 2 ? a.xrmp : a.xrmm;
+#pragma unroll
+    for (int c = 0; c < CL; c++) stage[l * CL + c] = o == 0 ? q[NQ - 1][c] : o == 1 ? vt[c] : q[o - 2][c];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CL; m++) {
+      const int k = l + 64 * m;
+      const int g = g0 + k;
+      if (k >= TAZW_HL && k < TAZW_HL + P && g < ns) {
+        int jo = g + sh;
+        if (jo >= ns) jo -= ns;
+        dst[base + jo] = stage[k];
+      }
+    }
+    __syncthreads();
   }
 }
-#undef FOR_CELLS
+
+template <int CL>
+__global__ __launch_bounds__(64) void k_tazw(taz_args a) {
+  tazw_body<CL>(a);
+}
+
+// cells per lane of the wave kernel for a ring of ns cells: CL = 4 or 5
+// (two waves per SIMD without spills: 174 and 206 VGPRs; CL >= 6 drops to
+// one wave or spills), the fewer total cells (chunks x 64 x CL), then the
+// longer chunks.  At 1024 x 3072: CL = 5, 10 chunks, 4 % halo overhead;
+// k_taz 143 us as one 256-thread workgroup per ring, 84 us as wave chunks.
+static int tazw_cells(int ns, int *nchunks) {
+  int best = 4, bestn = 0;
+  long bestc = -1;
+  for (int cl = 4; cl <= 5; cl++) {
+    const int p = 64 * cl - TAZW_HL - TAZW_HR;
+    const int n = (ns + p - 1) / p;
+    const long cells = (long)n * 64 * cl;
+    if (bestc < 0 || cells <= bestc) {
+      bestc = cells;
+      best = cl;
+      bestn = n;
+    }
+  }
+  *nchunks = bestn;
+  return best;
+}
 
 // ------------------------------------------------------------ k_fin ----
 // vrad recovery (Kernels.cu:1200-1205), the third damping pass, the
@@ -1539,19 +1635,11 @@ static int taz_cells(int ns) { return ring_cells(ns, TAZ_TPB); }
 // test per cell) when NSEC == TAZ_TPB * C and C <= 6 (for larger C it
 // needs more than 256 VGPRs)
 template <int C>
-static void taz_launch(bool fp32, dim3 g, dim3 blk, size_t shm, hipStream_t st, const taz_args &za) {
-  const bool full = C <= 6 && za.ns == TAZ_TPB * C;
-  if (fp32) {
-    if (full)
-      hipLaunchKernelGGL((k_taz32<C, true>), g, blk, shm, st, za);
-    else
-      hipLaunchKernelGGL((k_taz32<C, false>), g, blk, shm, st, za);
-  } else {
-    if (full)
-      hipLaunchKernelGGL((k_taz<C, true>), g, blk, shm, st, za);
-    else
-      hipLaunchKernelGGL((k_taz<C, false>), g, blk, shm, st, za);
-  }
+static void taz_launch(dim3 g, dim3 blk, size_t shm, hipStream_t st, const taz_args &za) {
+  if (C <= 6 && za.ns == TAZ_TPB * C)
+    hipLaunchKernelGGL((k_taz32<C, true>), g, blk, shm, st, za);
+  else
+    hipLaunchKernelGGL((k_taz32<C, false>), g, blk, shm, st, za);
 }
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
@@ -1680,17 +1768,26 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.ns = ns;
   za.fast = c->p.fast_transport;
   prof_mark(c);
-  const size_t shm = (fp32 ? 2 : 3) * (size_t)ns * sizeof(double);
+  const size_t shm = 2 * (size_t)ns * sizeof(double);
   const dim3 g(nr), blk(TAZ_TPB);
+  if (!fp32) {
+    int nch = 0;
+    const int cl = tazw_cells(ns, &nch);
+    const dim3 gw(nch, nr);
+    if (cl == 4)
+      hipLaunchKernelGGL(k_tazw<4>, gw, dim3(64), 0, c->stream, za);
+    else
+      hipLaunchKernelGGL(k_tazw<5>, gw, dim3(64), 0, c->stream, za);
+  } else
   switch (taz_cells(ns)) {
-    case 1: taz_launch<1>(fp32, g, blk, shm, c->stream, za); break;
-    case 2: taz_launch<2>(fp32, g, blk, shm, c->stream, za); break;
-    case 3: taz_launch<3>(fp32, g, blk, shm, c->stream, za); break;
-    case 4: taz_launch<4>(fp32, g, blk, shm, c->stream, za); break;
-    case 6: taz_launch<6>(fp32, g, blk, shm, c->stream, za); break;
-    case 8: taz_launch<8>(fp32, g, blk, shm, c->stream, za); break;
-    case 12: taz_launch<12>(fp32, g, blk, shm, c->stream, za); break;
-    default: taz_launch<16>(fp32, g, blk, shm, c->stream, za); break;
+    case 1: taz_launch<1>(g, blk, shm, c->stream, za); break;
+    case 2: taz_launch<2>(g, blk, shm, c->stream, za); break;
+    case 3: taz_launch<3>(g, blk, shm, c->stream, za); break;
+    case 4: taz_launch<4>(g, blk, shm, c->stream, za); break;
+    case 6: taz_launch<6>(g, blk, shm, c->stream, za); break;
+    case 8: taz_launch<8>(g, blk, shm, c->stream, za); break;
+    case 12: taz_launch<12>(g, blk, shm, c->stream, za); break;
+    default: taz_launch<16>(g, blk, shm, c->stream, za); break;
   }
   FG_LAUNCH_CHECK();
   // k_fin
