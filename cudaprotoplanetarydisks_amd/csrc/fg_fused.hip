@@ -23,6 +23,7 @@
 // self-gravity, per-DT CFL (everything else uses the per-routine path).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fg_cell.h"
@@ -461,11 +462,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
 
 // ------------------------------------------------------------ k_taz ----
 // OneWindTheta (TransportEuler.cu:177-194) + the vtheta half of
-// ComputeVelocities (Kernels.cu:1206-1207).  FP64: k_tazw below, wave
-// chunks.  FP32 (k_taz32 further down): one ring per 256-thread workgroup,
-// thread t owning sectors j = t + 256 k (k < C), neighbours exchanged
-// through LDS rings.  VMed comes from the ring-mean waves of k_trad.
-#define TAZ_TPB 256
+// ComputeVelocities (Kernels.cu:1206-1207) as wave chunks (k_tazw below),
+// FP64 and FP32 bulk mode.  VMed comes from the ring-mean waves of k_trad.
 
 struct taz_args {
   const double *vt;     // vtheta after k_src
@@ -492,6 +490,7 @@ struct taz_args {
 // overlap by those halos; each cell is output by exactly one chunk.
 #define TAZW_HL 5
 #define TAZW_HR 4
+#define TAZW_P32 (64 * 4 - TAZW_HL - TAZW_HR)  // cells output per FP32 chunk (CL = 4)
 
 // x of lane l-1 (lane 0 reads 0): DPP wave_shr:1 on each dword
 __device__ __forceinline__ double lane_prev(double x) {
@@ -507,6 +506,14 @@ __device__ __forceinline__ double lane_next(double x) {
   const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, true);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float lane_prev(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float lane_next(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 
 // QuantitiesAdvection (TransportEuler.cu:229-245) on a lane's CL cells with
@@ -564,20 +571,71 @@ __device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&V
 #undef V
 }
 
-// one wave per (chunk, ring): blockIdx.x = chunk, blockIdx.y = ring
-template <int CL>
-__device__ __forceinline__ void tazw_body(const taz_args &a) {
+// The FP32 bulk-mode sweep on a lane's CL cells: W, slopes, upwind values
+// and fluxes in FP32 with hardware reciprocals, each increment added to the
+// FP64 quantity
+template <int CL, bool UNIFORM>
+__device__ __forceinline__ void wave_sweep32(double (&q)[NQ][CL], const float (&Vc)[CL], float Vu,
+                                             const fg_dev_tables &t, int i, float dt) {
+  const float invdxtheta = (float)t.invdxtheta[i], dxtheta = (float)t.dxtheta[i];
+  const float dxrad = (float)((t.Rsup[i] - t.Rinf[i]) * (double)dt);
+  const float invsurf = (float)t.invSurf[i];
+#define V(c) (UNIFORM ? Vu : Vc[c])
+  float rs[CL], DS[CL];
+  {
+    float d[CL], D[CL];
+#pragma unroll
+    for (int c = 0; c < CL; c++) {
+      d[c] = (float)q[NQ - 1][c];
+      rs[c] = rcp_f(d[c] + 1e-20f);
+    }
+    const float Wm = lane_prev(d[CL - 1]), Wp = lane_next(d[0]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      D[c] = vl_theta_slope_f(c ? d[c - 1] : Wm, d[c], c < CL - 1 ? d[c + 1] : Wp, invdxtheta);
+    const float Dm = lane_prev(D[CL - 1]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      DS[c] = vl_theta_star_f(V(c), dt, c ? d[c - 1] : Wm, d[c], c ? D[c - 1] : Dm, D[c], dxtheta) * V(c) * dxrad;
+  }
+#pragma unroll
+  for (int qq = 0; qq < NQ; qq++) {
+    __builtin_amdgcn_sched_barrier(0);
+    float W[CL], D[CL], F[CL];
+#pragma unroll
+    for (int c = 0; c < CL; c++) W[c] = (float)q[qq][c] * rs[c];
+    const float Wm = lane_prev(W[CL - 1]), Wp = lane_next(W[0]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      D[c] = vl_theta_slope_f(c ? W[c - 1] : Wm, W[c], c < CL - 1 ? W[c + 1] : Wp, invdxtheta);
+    const float Dm = lane_prev(D[CL - 1]);
+#pragma unroll
+    for (int c = 0; c < CL; c++)
+      F[c] = vl_theta_star_f(V(c), dt, c ? W[c - 1] : Wm, W[c], c ? D[c - 1] : Dm, D[c], dxtheta) * DS[c];
+    const float Fp = lane_next(F[0]);
+#pragma unroll
+    for (int c = 0; c < CL; c++) q[qq][c] += (double)((F[c] - (c < CL - 1 ? F[c + 1] : Fp)) * invsurf);
+  }
+#undef V
+}
+
+// chunk `chunk` of ring i, one wave; F32 selects the FP32 bulk-mode sweeps
+template <int CL, bool F32>
+__device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   constexpr int N = 64 * CL, P = N - TAZW_HL - TAZW_HR;
   __shared__ double stage[N];
-  const int ns = a.ns, i = blockIdx.y, l = threadIdx.x;
-  const int g0 = (int)blockIdx.x * P - TAZW_HL;  // unwrapped index of the chunk's first cell
+  const int ns = a.ns, l = threadIdx.x;
+  const int g0 = chunk * P - TAZW_HL;  // unwrapped index of the chunk's first cell
   const fg_dev_tables &t = a.t;
   const long base = (long)i * ns;
   // loads: coalesced global reads into the chunk's LDS stage, then CL
   // consecutive cells per lane (a one-wave workgroup: the barriers only
   // order the LDS accesses of the wave)
-  double q[NQ][CL], vres[CL];
+  double q[NQ][CL];
+  using vel_t = typename std::conditional<F32, float, double>::type;
+  vel_t vres[CL];
   const double VMed = a.vmed[i];
+  double vt0[CL];
   // ring index of this lane's staged cell m: one modulo, then steps of 64
   int jl[CL];
   {
@@ -601,7 +659,7 @@ __device__ __forceinline__ void tazw_body(const taz_args &a) {
       if (qq < NQ)
         q[qq][c] = v;
       else
-        vres[c] = v - VMed;  // ComputeResidualsKernel
+        vt0[c] = v - VMed;  // ComputeResidualsKernel
     }
     __syncthreads();
   }
@@ -612,14 +670,17 @@ __device__ __forceinline__ void tazw_body(const taz_args &a) {
   const double Nround = floor(Ntilde + 0.5);
   const int nsh = (int)(long)Nround;
   double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
-  if (!a.fast) {
 #pragma unroll
-    for (int c = 0; c < CL; c++) vres[c] += vuni;
-    vuni = 0.0;
+  for (int c = 0; c < CL; c++) vres[c] = (vel_t)(a.fast ? vt0[c] : vt0[c] + vuni);
+  if (!a.fast) vuni = 0.0;
+  if (chunk == 0 && l == 0) a.nshift[i] = nsh;
+  if constexpr (F32) {
+    wave_sweep32<CL, false>(q, vres, 0.0f, t, i, (float)a.dt);
+    wave_sweep32<CL, true>(q, vres, (float)vuni, t, i, (float)a.dt);
+  } else {
+    wave_sweep<CL, false>(q, vres, 0.0, t, i, a.dt);
+    wave_sweep<CL, true>(q, vres, vuni, t, i, a.dt);
   }
-  if (blockIdx.x == 0 && l == 0) a.nshift[i] = nsh;
-  wave_sweep<CL, false>(q, vres, 0.0, t, i, a.dt);
-  wave_sweep<CL, true>(q, vres, vuni, t, i, a.dt);
   // vtheta from the angular momenta (Kernels.cu:1206-1207): cells j - 1
   // and j of the unshifted ring
   const double Rm = t.Rmed[i];
@@ -658,9 +719,10 @@ __device__ __forceinline__ void tazw_body(const taz_args &a) {
   }
 }
 
-template <int CL>
+// blockIdx.x = chunk, blockIdx.y = ring
+template <int CL, bool F32>
 __global__ __launch_bounds__(64) void k_tazw(taz_args a) {
-  tazw_body<CL>(a);
+  tazw_body<CL, F32>(a, blockIdx.y, blockIdx.x);
 }
 
 // cells per lane of the wave kernel for a ring of ns cells: CL = 4 or 5
@@ -1360,124 +1422,6 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
 
-// Azimuthal sweep in FP32 (azimuthal_sweep above); q stays FP64.  Each
-// thread keeps its own cells' values and slopes in registers and reads
-// only the neighbours' from the LDS rings W, D, F.
-template <int C, bool FULL, bool UNIFORM>
-__device__ __forceinline__ void azimuthal_sweep32(double (&q)[NQ][C], const float (&Vc)[C], float Vu, float *W,
-                                                  float *D, float *Fl, int ns, int i, const fg_dev_tables &t,
-                                                  float dt) {
-  const float invdxtheta = (float)t.invdxtheta[i], dxtheta = (float)t.dxtheta[i];
-  const float dxrad = (float)((t.Rsup[i] - t.Rinf[i]) * (double)dt);
-  const float invsurf = (float)t.invSurf[i];
-  float rs[C], DS[C], wk[C], dk[C];
-#define V(k) (UNIFORM ? Vu : Vc[k])
-#define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
-#define JM(j) ((j) == 0 ? ns - 1 : (j) - 1)
-#define JP(j) ((j) + 1 == ns ? 0 : (j) + 1)
-  FOR_CELLS(k, j) {
-    const float d = (float)q[NQ - 1][k];
-    rs[k] = rcp_f(d + 1e-20f);
-    wk[k] = d;
-    W[j] = d;
-  }
-  __syncthreads();
-  FOR_CELLS(k, j) {
-    dk[k] = vl_theta_slope_f(W[JM(j)], wk[k], W[JP(j)], invdxtheta);
-    D[j] = dk[k];
-  }
-  __syncthreads();
-  FOR_CELLS(k, j) DS[k] = vl_theta_star_f(V(k), dt, W[JM(j)], wk[k], D[JM(j)], dk[k], dxtheta) * V(k) * dxrad;
-  __syncthreads();
-#pragma unroll
-  for (int qq = 0; qq < NQ; qq++) {
-    FOR_CELLS(k, j) {
-      wk[k] = (float)q[qq][k] * rs[k];
-      W[j] = wk[k];
-    }
-    __syncthreads();
-    FOR_CELLS(k, j) {
-      dk[k] = vl_theta_slope_f(W[JM(j)], wk[k], W[JP(j)], invdxtheta);
-      D[j] = dk[k];
-    }
-    __syncthreads();
-    float F[C];
-    FOR_CELLS(k, j) {
-      F[k] = vl_theta_star_f(V(k), dt, W[JM(j)], wk[k], D[JM(j)], dk[k], dxtheta) * DS[k];
-      Fl[j] = F[k];
-    }
-    __syncthreads();
-    FOR_CELLS(k, j) q[qq][k] += (double)((F[k] - Fl[JP(j)]) * invsurf);
-  }
-  __syncthreads();
-#undef V
-#undef FOR_CELLS
-#undef JM
-#undef JP
-}
-
-// one ring; lds holds 2 * NSEC doubles (three float rings during the
-// sweeps, two double rings for the vtheta recovery)
-template <int C, bool FULL>
-__device__ __forceinline__ void taz32_ring(const taz_args &a, int i, double *lds) {
-  float *Wf = (float *)lds, *Df = Wf + a.ns, *Ff = Df + a.ns;
-  double *W = lds, *D = lds + a.ns;
-  const int ns = a.ns, tid = threadIdx.x;
-  const fg_dev_tables &t = a.t;
-  const long base = (long)i * ns;
-  const double VMed = a.vmed[i];
-  double q[NQ][C];
-  float vres[C];
-#define FOR_CELLS(k, j) \
-  _Pragma("unroll") for (int k = 0, j = threadIdx.x; k < C; k++, j += TAZ_TPB) if (FULL || j < ns)
-  // FARGO residual phase in FP64 (Kernels.cu:1084-1126)
-  const double invdt = 1.0 / a.dt;
-  const double dpinvns = 2.0 * FG_PI / (double)ns;
-  const double Ntilde = VMed * t.invRmed[i] * a.dt * (double)ns / 2.0 / FG_PI;
-  const double Nround = floor(Ntilde + 0.5);
-  const int nsh = (int)(long)Nround;
-  double vuni = (Ntilde - Nround) * t.Rmed[i] * invdt * dpinvns;
-  FOR_CELLS(k, j) {
-    double vr = a.vt[base + j] - VMed;
-    if (!a.fast) vr += vuni;
-    vres[k] = (float)vr;
-#pragma unroll
-    for (int qq = 0; qq < NQ; qq++) q[qq][k] = a.q[qq][base + j];
-  }
-  if (!a.fast) vuni = 0.0;
-  if (tid == 0) a.nshift[i] = nsh;
-  azimuthal_sweep32<C, FULL, false>(q, vres, 0.0f, Wf, Df, Ff, ns, i, t, (float)a.dt);
-  azimuthal_sweep32<C, FULL, true>(q, vres, (float)vuni, Wf, Df, Ff, ns, i, t, (float)a.dt);
-  // shift folded into the stores, as in k_taz
-  int sh = nsh % ns;
-  if (sh < 0) sh += ns;
-  FOR_CELLS(k, j) {
-    W[j] = q[2][k];
-    D[j] = q[NQ - 1][k];
-  }
-  __syncthreads();
-  const double Rm = t.Rmed[i];
-  FOR_CELLS(k, j) {
-    int jm = j == 0 ? ns - 1 : j - 1;
-    double s = q[NQ - 1][k];
-    double v = (W[jm] + q[3][k]) / (s + D[jm] + 1e-15) / Rm - Rm * a.omega_frame;
-    int jo = j + sh;
-    if (jo >= ns) jo -= ns;
-    a.xd[base + jo] = s;
-    a.xvt[base + jo] = v;
-    a.xrmp[base + jo] = q[0][k];
-    a.xrmm[base + jo] = q[1][k];
-  }
-#undef FOR_CELLS
-}
-
-template <int C, bool FULL>
-__global__ __launch_bounds__(TAZ_TPB) void k_taz32(taz_args a) {
-  extern __shared__ double lds_taz32[];
-  taz32_ring<C, FULL>(a, blockIdx.x, lds_taz32);
-}
-
 // ----------------------------------------------------- entry points ----
 #define PROF_KERNELS 6
 
@@ -1621,27 +1565,6 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   return FG_OK;
 }
 
-// cells per thread of k_taz: the smallest instantiated C with
-// (threads per ring) * C >= NSEC
-static int ring_cells(int ns, int tpb) {
-  const int cs[] = {1, 2, 3, 4, 6, 8, 12, 16};
-  for (int c : cs)
-    if (tpb * c >= ns) return c;
-  return 0;
-}
-static int taz_cells(int ns) { return ring_cells(ns, TAZ_TPB); }
-
-// k_taz / k_taz32 with C cells per thread; the FULL variant (no bounds
-// test per cell) when NSEC == TAZ_TPB * C and C <= 6 (for larger C it
-// needs more than 256 VGPRs)
-template <int C>
-static void taz_launch(dim3 g, dim3 blk, size_t shm, hipStream_t st, const taz_args &za) {
-  if (C <= 6 && za.ns == TAZ_TPB * C)
-    hipLaunchKernelGGL((k_taz32<C, true>), g, blk, shm, st, za);
-  else
-    hipLaunchKernelGGL((k_taz32<C, false>), g, blk, shm, st, za);
-}
-
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
                                    double DeltaT_next, int want_temperature) {
@@ -1768,26 +1691,19 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.ns = ns;
   za.fast = c->p.fast_transport;
   prof_mark(c);
-  const size_t shm = 2 * (size_t)ns * sizeof(double);
-  const dim3 g(nr), blk(TAZ_TPB);
   if (!fp32) {
     int nch = 0;
     const int cl = tazw_cells(ns, &nch);
     const dim3 gw(nch, nr);
     if (cl == 4)
-      hipLaunchKernelGGL(k_tazw<4>, gw, dim3(64), 0, c->stream, za);
+      hipLaunchKernelGGL((k_tazw<4, false>), gw, dim3(64), 0, c->stream, za);
     else
-      hipLaunchKernelGGL(k_tazw<5>, gw, dim3(64), 0, c->stream, za);
-  } else
-  switch (taz_cells(ns)) {
-    case 1: taz_launch<1>(g, blk, shm, c->stream, za); break;
-    case 2: taz_launch<2>(g, blk, shm, c->stream, za); break;
-    case 3: taz_launch<3>(g, blk, shm, c->stream, za); break;
-    case 4: taz_launch<4>(g, blk, shm, c->stream, za); break;
-    case 6: taz_launch<6>(g, blk, shm, c->stream, za); break;
-    case 8: taz_launch<8>(g, blk, shm, c->stream, za); break;
-    case 12: taz_launch<12>(g, blk, shm, c->stream, za); break;
-    default: taz_launch<16>(g, blk, shm, c->stream, za); break;
+      hipLaunchKernelGGL((k_tazw<5, false>), gw, dim3(64), 0, c->stream, za);
+  } else {
+    // FP32: CL = 4 (112 VGPRs, four waves per SIMD); 113 -> 56 us at
+    // 1024 x 3072 against one 256-thread workgroup per ring
+    const dim3 gw((ns + TAZW_P32 - 1) / TAZW_P32, nr);
+    hipLaunchKernelGGL((k_tazw<4, true>), gw, dim3(64), 0, c->stream, za);
   }
   FG_LAUNCH_CHECK();
   // k_fin
@@ -1886,12 +1802,14 @@ __global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__res
   }
 }
 
-template <int C>
-__global__ __launch_bounds__(TAZ_TPB) void k_batch_taz(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
-  extern __shared__ double lds_btaz[];
+// one chunk of one ring per item: y = ring * chunks + chunk
+__global__ __launch_bounds__(64) void k_batch_taz(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) taz32_ring<C, false>(D[w.x].taz, w.y, lds_btaz);
+    if (D[w.x].active) {
+      const int nch = (D[w.x].taz.ns + TAZW_P32 - 1) / TAZW_P32;
+      tazw_body<4, true>(D[w.x].taz, w.y / nch, w.y % nch);
+    }
     __syncthreads();
   }
 }
@@ -1941,7 +1859,7 @@ struct fg_batch_dev {
   int4 *it_src;
   int2 *it_trad, *it_taz, *it_fin, *it_force;
   int n_src, n_trad, n_taz, n_fin, n_force;
-  int max_th, max_ns, taz_c;
+  int max_th, max_ns;
   hipGraph_t graph;
   hipGraphExec_t exec;
   int ready;  // items built and graph instantiated
@@ -2032,17 +1950,13 @@ static fg_status batch_build(fg_batch_dev *b) {
     const int nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
     const int nblk = nr + nstrips * ((nr + band - 1) / band);
     for (int q = 0; q < nblk; q++) trad.push_back(make_int2(k, q));
+    const int nch = (ns + TAZW_P32 - 1) / TAZW_P32;
     for (int i = 0; i < nr; i++) {
-      taz.push_back(make_int2(k, i));
+      for (int ch = 0; ch < nch; ch++) taz.push_back(make_int2(k, i * nch + ch));
       fin.push_back(make_int2(k, i));
     }
     const int B = red_blocks((long)nr * ns);
     for (int q = 0; q < B; q++) force.push_back(make_int2(k, q));
-  }
-  b->taz_c = taz_cells(b->max_ns);
-  if (!b->taz_c) {
-    fg_set_error("batch: NSEC %d too large", b->max_ns);
-    return FG_E_UNSUPPORTED;
   }
   b->n_src = (int)src.size();
   b->n_trad = (int)trad.size();
@@ -2064,21 +1978,7 @@ static fg_status batch_build(fg_batch_dev *b) {
   hipLaunchKernelGGL(k_batch_src, dim3(b->n_src > 0 ? b->n_src : 1), dim3(SRC_TPB), src32_lds_bytes(b->max_th), st,
                      b->d_disk, b->it_src, b->n_src);
   hipLaunchKernelGGL(k_batch_trad, grid(b->n_trad), dim3(TRAD_TPB), 0, st, b->d_disk, b->it_trad, b->n_trad);
-  const size_t shm_taz = 2 * (size_t)b->max_ns * sizeof(double);
-  const dim3 gt = grid(b->n_taz), bt(TAZ_TPB);
-  const batch_disk *D = b->d_disk;
-  const int2 *IT = b->it_taz;
-  const int NT = b->n_taz;
-  switch (b->taz_c) {
-    case 1: hipLaunchKernelGGL(k_batch_taz<1>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 2: hipLaunchKernelGGL(k_batch_taz<2>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 3: hipLaunchKernelGGL(k_batch_taz<3>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 4: hipLaunchKernelGGL(k_batch_taz<4>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 6: hipLaunchKernelGGL(k_batch_taz<6>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 8: hipLaunchKernelGGL(k_batch_taz<8>, gt, bt, shm_taz, st, D, IT, NT); break;
-    case 12: hipLaunchKernelGGL(k_batch_taz<12>, gt, bt, shm_taz, st, D, IT, NT); break;
-    default: hipLaunchKernelGGL(k_batch_taz<16>, gt, bt, shm_taz, st, D, IT, NT); break;
-  }
+  hipLaunchKernelGGL(k_batch_taz, grid(b->n_taz), dim3(64), 0, st, b->d_disk, b->it_taz, b->n_taz);
   hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
                      b->it_fin, b->n_fin);
   hipLaunchKernelGGL(k_batch_close, grid(b->n), dim3(256), 0, st, b->d_disk, b->d_res, b->n);
