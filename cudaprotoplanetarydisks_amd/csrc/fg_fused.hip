@@ -179,8 +179,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // ---- S1: Substep1 (Kernels.cu:13-53) on rows [1, KHR), cols [1, KHC-1) ----
   double regA[KPT], regB[KPT];
   {
-    int k = 0;
-    for (int q = tid; q < KHR * KHC; q += SRC_TPB, k++) {
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+      const int q = tid + k * SRC_TPB;
+      if (q >= KHR * KHC) break;
       int r = q / KHC, cc = q - r * KHC;
       int i = GI(r);
       double vri = 0.0, vti = 0.0;
@@ -213,8 +215,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       }
     }
     __syncthreads();
-    k = 0;
-    for (int q = tid; q < KHR * KHC; q += SRC_TPB, k++) {
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+      const int q = tid + k * SRC_TPB;
+      if (q >= KHR * KHC) break;
       int r = q / KHC, cc = q - r * KHC;
       sA[r][cc] = regA[k];  // VradInt (0 on rows 0 and NRAD, as never written)
       sB[r][cc] = regB[k];  // VthetaInt
@@ -225,7 +229,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // ---- S2: viscous stress (Kernels.cu:733-771) on rows [1, KHR-1), cols [2, KHC-2) ----
   const double invdphi = t.invdphi;
   const double onethird = 1.0 / 3.0;
-  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < KPT; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
     if (r < 1 || r >= KHR - 1 || cc < 2 || cc >= KHC - 2 || i < 0 || i >= nr) continue;
@@ -246,7 +253,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __syncthreads();
 
   // ---- S3: viscous velocity update (Kernels.cu:118-150), in place ----
-  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < KPT; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
     if (r < 2 || r >= KHR - 1 || cc < 3 || cc >= KHC - 3 || i < 1 || i >= nr) continue;
@@ -263,7 +273,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __syncthreads();
 
   // ---- S4: artificial pressure (Kernels.cu:409-434) on rows [3, KHR-1), cols [3, KHC-4) ----
-  for (int q = tid; q < KHR * KHC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < KPT; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
     if (r < 3 || r >= KHR - 1 || cc < 3 || cc >= KHC - 4 || i < 0 || i >= nr) continue;
@@ -273,7 +286,10 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __syncthreads();
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2, store ----
-  for (int q = tid; q < KTH * KTW; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < (KTH * KTW + SRC_TPB - 1) / SRC_TPB; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= KTH * KTW) break;
     int r = q / KTW + HALO_R, cc = q % KTW + HALO_C;
     int i = GI(r), j = j0 + cc - HALO_C;
     if (i > nr || j >= ns) continue;
@@ -947,7 +963,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     double invdt2 = fabs(vr) / dxrad;
     double invdt3 = fabs(vres) / dxtheta;
     double dvr = vrn - vr;
-    double dvt = ring[(j + 1) % ns] - ring[j];
+    double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
     dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
     dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
     double x = dvr / dxrad, y = dvt / dxtheta;
@@ -1045,6 +1061,21 @@ __device__ __forceinline__ void src32_adjust(const src32_args &a, int i, const d
 }
 
 #define SRC32_OWN_MAX (SRC32_MAX_TH * TW / SRC_TPB)  // own cells per thread
+
+// FP64 state of own cell q of the tile (0 outside the grid)
+__device__ __forceinline__ void own_load(const src32_args &a, int i0, int j0, int th, int q, double &s, double &vr,
+                                         double &vt) {
+  s = vr = vt = 0.0;
+  if (q < th * TW) {
+    const int i = i0 + q / TW, j = j0 + q % TW;
+    if (i <= a.nr && j < a.ns) {
+      const long l = (long)i * a.ns + j;
+      s = a.dens[l];
+      vr = a.vr[l];
+      vt = a.vt[l];
+    }
+  }
+}
 
 // ROWC: the damping constants of the tile's rings are computed once per
 // ring into LDS; otherwise once per cell
@@ -1186,7 +1217,10 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   // ---- S2: viscous stress (Kernels.cu:733-771) ----
   const float invdphi = (float)t.invdphi;
   const float onethird = 1.0f / 3.0f;
-  for (int q = tid; q < hr * HC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     if (r < 1 || r >= hr - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
@@ -1212,7 +1246,10 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   __syncthreads();
 
   // ---- S3: viscous velocity update (Kernels.cu:118-150) ----
-  for (int q = tid; q < hr * HC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     if (r < 2 || r >= hr - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
@@ -1237,7 +1274,10 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   __syncthreads();
 
   // ---- S4: artificial pressure (Kernels.cu:409-434) ----
-  for (int q = tid; q < hr * HC; q += SRC_TPB) {
+#pragma unroll
+  for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
+    const int q = tid + kq * SRC_TPB;
+    if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     if (r < 3 || r >= hr - 1 || cc < 3 || cc >= HC - 4 || i < 0 || i >= nr) continue;
@@ -1249,19 +1289,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
   double os[SRC32_OWN_MAX], ovr[SRC32_OWN_MAX], ovt[SRC32_OWN_MAX];
 #pragma unroll
-  for (int k = 0; k < SRC32_OWN_MAX; k++) {
-    const int q = tid + k * SRC_TPB;
-    os[k] = ovr[k] = ovt[k] = 0.0;
-    if (q < th * TW) {
-      const int i = GI(q / TW + HALO_R), j = j0 + q % TW;
-      if (i <= nr && j < ns) {
-        const long l = (long)i * ns + j;
-        os[k] = a.dens[l];
-        ovr[k] = a.vr[l];
-        ovt[k] = a.vt[l];
-      }
-    }
-  }
+  for (int k = 0; k < SRC32_OWN_MAX; k++) own_load(a, i0, j0, th, tid + k * SRC_TPB, os[k], ovr[k], ovt[k]);
 #pragma unroll
   for (int k = 0; k < SRC32_OWN_MAX; k++) {
     const int q = tid + k * SRC_TPB;
