@@ -28,6 +28,7 @@
 
 #include "fg_cell.h"
 #include "fg_internal.h"
+#include "fg_reduce.h"
 
 #define TH 16
 #define TW 64
@@ -842,36 +843,6 @@ __device__ __forceinline__ void force_rings_final(const double *fpart, int nr, d
 
 __global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr, double *out2) {
   force_rings_final(fpart, nr, out2);
-}
-
-// Left-to-right sum of ring[0..n) in one lane, the order of the
-// reference's one-thread ring loops (Kernels.cu:1276-1279).  Values come
-// from LDS in blocks of 16, the next block's loads issued before the
-// current block's adds, so the LDS latency overlaps the dependent adds.
-__device__ __forceinline__ double seq_sum_lds(const double *ring, int n) {
-  double m = 0.0;
-  int j = 0;
-  if (n >= 32) {
-    double a[16], b[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) a[q] = ring[q];
-    for (; j + 32 <= n; j += 32) {
-#pragma unroll
-      for (int q = 0; q < 16; q++) b[q] = ring[j + 16 + q];
-      __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the adds
-#pragma unroll
-      for (int q = 0; q < 16; q++) m += a[q];
-      // the next a block, or (past the last full pair) a harmless re-read
-      const int jn = (j + 48 <= n) ? j + 32 : j;
-#pragma unroll
-      for (int q = 0; q < 16; q++) a[q] = ring[jn + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 16; q++) m += b[q];
-    }
-  }
-  for (; j < n; j++) m += ring[j];
-  return m;
 }
 
 // one ring; ring holds NSEC doubles (the ring's final vtheta)
@@ -1774,7 +1745,6 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
 // phases and the reductions, and copies back each disk's CFL count and
 // star force.  The per-disk arithmetic is the single-disk FP32 code, so
 // a disk gives the same bits batched or alone.
-#include "fg_reduce.h"
 
 struct batch_disk {
   src32_args src;

@@ -21,6 +21,20 @@
 #define RED_TPB 256
 #define RED_CH 16
 
+// ring and sector of flattened cell e: a 32-bit division (one float
+// reciprocal and corrections) below 2^31 cells, the emulated 64-bit one
+// beyond
+__device__ __forceinline__ void cell_ij(long e, int ns, int &i, int &j) {
+  if (e < 0x80000000L) {
+    const unsigned ue = (unsigned)e, q = ue / (unsigned)ns;
+    i = (int)q;
+    j = (int)(ue - q * (unsigned)ns);
+  } else {
+    i = (int)(e / ns);
+    j = (int)(e - (long)i * ns);
+  }
+}
+
 static inline int red_blocks(long n) {
   long p = 1;
   while (p < n) p <<= 1;
@@ -35,8 +49,8 @@ struct force_src {
   double x, y, rs;
   int ns;
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
-    int i = (int)(e / ns);
-    int j = (int)(e - (long)i * ns);
+    int i, j;
+    cell_ij(e, ns, i, j);
     double cellmass = Surf[i] * dens[e];
     double dx = Rmed[i] * cosc[j] - x;
     double dy = Rmed[i] * sinc[j] - y;
@@ -58,8 +72,8 @@ struct hill_force_src {
   double x, y, rs, a, rh;
   int ns, dimfxy;
   __device__ __forceinline__ void operator()(long e, int k, double v[4]) const {
-    int i = (int)(e / ns);
-    int j = (int)(e - (long)i * ns);
+    int i, j;
+    cell_ij(e, ns, i, j);
     double cellmass = Surf[i] * dens[e];
     double dx = Rmed[i] * cosc[j] - x;
     double dy = Rmed[i] * sinc[j] - y;
@@ -89,8 +103,8 @@ struct cpm_src {
   double xpl, ypl, hill;
   int ns;
   __device__ __forceinline__ void operator()(long e, double &m, double &zero) const {
-    int i = (int)(e / ns);
-    int j = (int)(e - (long)i * ns);
+    int i, j;
+    cell_ij(e, ns, i, j);
     double dx = Rmed[i] * cosc[j] - xpl;
     double dy = Rmed[i] * sinc[j] - ypl;
     double dist = sqrt(dx * dx + dy * dy);
@@ -109,15 +123,19 @@ struct array_src {
 
 // block blk of B (a RED_TPB-thread workgroup); partial[blk] and
 // partial[B + blk] receive the x and y block sums
-template <class Src>
+// CH summand rows are staged per round (CH * 64 / RED_TPB per thread, all
+// evaluated before the round's barrier)
+template <class Src, int CH = RED_CH>
 __device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int blk, double *partial) {
-  __shared__ double vx[RED_CH][64];
-  __shared__ double vy[RED_CH][64];
+  __shared__ double vx[CH][64];
+  __shared__ double vy[CH][64];
   const long G = 64L * B;
   const long b64 = 64L * blk;
   double acc = 0.0;
-  for (long k0 = 0; b64 + k0 * G < n; k0 += RED_CH) {
-    for (int c = threadIdx.x; c < RED_CH * 64; c += RED_TPB) {
+  for (long k0 = 0; b64 + k0 * G < n; k0 += CH) {
+#pragma unroll
+    for (int c0 = 0; c0 < CH * 64 / RED_TPB; c0++) {
+      const int c = threadIdx.x + c0 * RED_TPB;
       int kk = c >> 6, lane = c & 63;
       long e = b64 + (k0 + kk) * G + lane;
       double fx = 0.0, fy = 0.0;
@@ -129,7 +147,7 @@ __device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int
     if (threadIdx.x < 64) {
       const int t = threadIdx.x & 31;
       const bool isy = threadIdx.x >= 32;
-      for (int kk = 0; kk < RED_CH; kk++) {
+      for (int kk = 0; kk < CH; kk++) {
         long e = b64 + (k0 + kk) * G + t;
         if (e < n) {
           acc += isy ? vy[kk][t] : vx[kk][t];
@@ -183,6 +201,36 @@ __device__ __forceinline__ void dreduce_block4(const Src &src, int k, long n, in
   }
 }
 
+// Left-to-right sum of ring[0..n) in one lane, the order of the
+// reference's one-thread ring loops (Kernels.cu:1276-1279).  Values come
+// from LDS in blocks of 16, the next block's loads issued before the
+// current block's adds, so the LDS latency overlaps the dependent adds.
+__device__ __forceinline__ double seq_sum_lds(const double *ring, int n) {
+  double m = 0.0;
+  int j = 0;
+  if (n >= 32) {
+    double a[16], b[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) a[q] = ring[q];
+    for (; j + 32 <= n; j += 32) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) b[q] = ring[j + 16 + q];
+      __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the adds
+#pragma unroll
+      for (int q = 0; q < 16; q++) m += a[q];
+      // the next a block, or (past the last full pair) a harmless re-read
+      const int jn = (j + 48 <= n) ? j + 32 : j;
+#pragma unroll
+      for (int q = 0; q < 16; q++) a[q] = ring[jn + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; q++) m += b[q];
+    }
+  }
+  for (; j < n; j++) m += ring[j];
+  return m;
+}
+
 // host loop of DeviceReduce (Kernels.cu:385-390).  With add_inner, the
 // result is fx_inner + fx_outer of SideEuler.cu:144 where the inner sum
 // is the DeviceReduce of an all-zero array (+0.0).
@@ -194,8 +242,7 @@ __device__ __forceinline__ void dreduce_final(const double *partial, int B, doub
   __syncthreads();
   int lane = threadIdx.x;
   if (lane >= nv) return;
-  double s = 0.0;
-  for (int b = 0; b < B; b++) s += p[lane * B + b];
+  double s = seq_sum_lds(p + lane * B, B);
   if (add_inner) s = 0.0 + s;
   out[lane] = s;
 }

@@ -3,9 +3,9 @@
 // boundary, whose ring means are such sums.
 #include "fg_reduce.h"
 
-template <class Src>
+template <class Src, int CH>
 __global__ __launch_bounds__(RED_TPB) void k_dreduce_partial(Src src, long n, int B, double *partial) {
-  dreduce_block(src, n, B, blockIdx.x, partial);
+  dreduce_block<Src, CH>(src, n, B, blockIdx.x, partial);
 }
 
 __global__ void k_dreduce_final(const double *partial, int B, double *out, int two, int add_inner) {
@@ -15,7 +15,10 @@ __global__ void k_dreduce_final(const double *partial, int B, double *out, int t
 template <class Src>
 static fg_status dreduce(fg_ctx *c, Src src, long n, double *out_dev, int two, int add_inner) {
   int B = red_blocks(n);
-  hipLaunchKernelGGL(k_dreduce_partial<Src>, dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B, c->red_partial);
+  // 16 summand rows per round: 24 or 48 measured the same or slower
+  // (fewer resident workgroups)
+  hipLaunchKernelGGL((k_dreduce_partial<Src, RED_CH>), dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B,
+                     c->red_partial);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_dreduce_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, out_dev, two, add_inner);
   FG_LAUNCH_CHECK();
