@@ -780,13 +780,6 @@ struct fin_args {
   int nr, ns, evanescent, fast;
 };
 
-__device__ __forceinline__ double recover_vr(const fin_args &a, int i, int j) {
-  const int ns = a.ns;
-  if (i == 0) return 0.0;
-  if (i >= a.nr) return 0.0;
-  long l = (long)i * ns + j;
-  return (a.xrmp[l - ns] + a.xrmm[l]) / (a.xd[l] + a.xd[l - ns] + 1e-20);
-}
 
 // Neumaier-compensated sums (value s, correction c) of the FP32 mode's
 // star force: per thread in cell order, then a fixed shuffle tree per
@@ -845,7 +838,10 @@ __global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr
   force_rings_final(fpart, nr, out2);
 }
 
-// one ring; ring holds NSEC doubles (the ring's final vtheta)
+// one ring; ring holds NSEC doubles (the ring's final vtheta).  FIN_B
+// cells per thread have their loads issued together (4 for the single-disk
+// kernel; 1 in the batched one, which would otherwise lose a wave per SIMD)
+template <int FIN_B>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[256];
   __shared__ double s_vmoy;
@@ -859,9 +855,37 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
   double fsx = 0.0, fcx = 0.0, fsy = 0.0, fcy = 0.0;
   const double Rm = t.Rmed[i], surf = t.Surf[i];
-  for (int j = tid; j < ns; j += 256) {
-    long l = base + j;
-    double vr = recover_vr(a, i, j), vt = a.xvt[l], s = a.xd[l];
+  // star at the origin (x = y = 0, no smoothing): the cell's distance is
+  // Rmed, so 1/d^3 is one value per ring (FP32 mode only; the FP64 path
+  // sums the reference's per-cell expression in DeviceReduce order)
+  const double inv3 = 1.0 / (Rm * Rm) / Rm;
+  const bool radial = i > 0 && i < nr;  // vrad recovery reads rings i-1 and i
+  // FIN_B cells per thread per batch: all their loads first, then the math
+  for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
+    double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B];
+#pragma unroll
+    for (int u = 0; u < FIN_B; u++) {
+      const int j = j0 + u * 256;
+      lrp[u] = lrm[u] = ld[u] = ldm[u] = lvt[u] = lp[u] = 0.0;
+      if (j < ns) {
+        const long l = base + j;
+        if (radial) {
+          lrp[u] = a.xrmp[l - ns];
+          lrm[u] = a.xrmm[l];
+          ldm[u] = a.xd[l - ns];
+        }
+        ld[u] = a.xd[l];
+        lvt[u] = a.xvt[l];
+        if (a.temp) lp[u] = a.press[l];
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < FIN_B; u++) {
+    const int j = j0 + u * 256;
+    if (j >= ns) break;
+    const long l = base + j;
+    // ComputeVelocities, Kernels.cu:1200-1205 (0 on ring 0 and the ghost ring)
+    double vr = radial ? (lrp[u] + lrm[u]) / (ld[u] + ldm[u] + 1e-20) : 0.0, vt = lvt[u], s = ld[u];
     if (damp_i) {
       vr = damp_toward(vr, lam_i, vr0_i);
       vt = damp_toward(vt, lam_i, vt0_i);
@@ -869,8 +893,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     }
     if (a.fpart) {  // gravity of the cell on the star (Kernels.cu:703-727 at x = y = 0)
       const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
-      const double dist2 = dx * dx + dy * dy;
-      const double inv3 = 1.0 / dist2 / sqrt(dist2);
       const double cm = surf * s;
       kadd(fsx, fcx, cm * dx * inv3);
       kadd(fsy, fcy, cm * dy * inv3);
@@ -879,12 +901,13 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
-    if (a.temp) a.temp[l] = a.press[l] / s;
+    if (a.temp) a.temp[l] = lp[u] / s;
     if (i == nr - 1) {  // ghost ring after transport: all zero
       a.dens[l + ns] = 0.0;
       a.vr[l + ns] = 0.0;
       a.vt[l + ns] = 0.0;
     }
+  }
   }
   if (a.fpart) {
     __shared__ double fw[4][4];
@@ -924,10 +947,30 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
   const double vm = s_vmoy, cs = t.cs_iso[i];
   double best = 1e30;
-  for (int j = tid; j < ns; j += 256) {
-    long l = base + j;
-    double vr = a.vr[l];  // written above by this thread
-    double vrn = recover_vr(a, i + 1, j);
+  const bool radial_n = i + 1 < nr;  // vrad of ring i+1 (0 on the ghost ring)
+  for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
+    double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B];
+#pragma unroll
+    for (int u = 0; u < FIN_B; u++) {
+      const int j = j0 + u * 256;
+      pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = 0.0;
+      if (j < ns) {
+        const long l = base + j;
+        pvr[u] = a.vr[l];  // written above by this thread
+        if (radial_n) {
+          prp[u] = a.xrmp[l];
+          prm[u] = a.xrmm[l + ns];
+          pd1[u] = a.xd[l + ns];
+          pd0[u] = a.xd[l];
+        }
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < FIN_B; u++) {
+    const int j = j0 + u * 256;
+    if (j >= ns) break;
+    double vr = pvr[u];
+    double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
     if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
     double vres = a.fast ? (ring[j] - vm) : ring[j];
     double invdt1 = cs / mindx;
@@ -942,6 +985,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     invdt4 *= 4.0 * 1.41 * 1.41;
     double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
     if (dtc < best) best = dtc;
+  }
   }
   red[tid] = best;
   __syncthreads();
@@ -961,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
   const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
   if (i >= a.nr) return;
-  fin_ring(a, i, lds_fin);
+  fin_ring<4>(a, i, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -1816,7 +1860,7 @@ __global__ __launch_bounds__(256) void k_batch_fin(const batch_disk *__restrict_
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) fin_ring(D[w.x].fin, w.y, lds_bfin);
+    if (D[w.x].active) fin_ring<1>(D[w.x].fin, w.y, lds_bfin);
     __syncthreads();
   }
 }
