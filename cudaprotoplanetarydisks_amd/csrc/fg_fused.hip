@@ -8,10 +8,11 @@
 //           viscosity, velocity update, second damping pass
 //           state A -> state B
 //   k_trad  column-marching radial van Leer sweep of the five transported
-//           quantities (each slope and flux evaluated once)
+//           quantities (each slope and flux evaluated once), plus one wave
+//           per ring for VMed (sequential ring sum)
 //           state B -> R (Sigma, RadMomP/M, ThetaMomP/M)
-//   k_taz   one ring per workgroup: VMed (sequential sum), FARGO residual
-//           and integer shift, both azimuthal sweeps, the shift, vtheta
+//   k_tazw  one wave per contiguous chunk of a ring: FARGO residual and
+//           integer shift, both azimuthal sweeps, the shift, vtheta
 //           recovery; R -> X (Sigma, vtheta, RadMomP, RadMomM)
 //   k_fin   one ring per workgroup: vrad recovery, third damping pass,
 //           temperature, and the next substep's CFL ring minimum
