@@ -370,6 +370,40 @@ __device__ void ring_mean_wave(const double *v, int ns, double *out) {
   if (lane == 0) *out = sum / (double)ns;
 }
 
+// x of lane `lane` (uniform), as scalars
+__device__ __forceinline__ double lane_get(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Ring tables of the rings a column strip marches over, one ring per lane
+// (lane r holds ring r0 + r, clamped into the nr + 1 entries), read back
+// with v_readlane.  Reading the tables from memory inside the ring loop
+// would make each iteration wait for that load and, as a wave's vector
+// memory operations complete in order, for the next ring's field loads
+// issued before it, leaving nothing in flight across the arithmetic.
+struct ring_lanes {
+  int r0;
+  double Rmed, idm, Rinf, isurf;
+  __device__ __forceinline__ void load(const fg_dev_tables &t, int base, int nr) {
+    r0 = base;
+    const int r = min(max(base + (int)threadIdx.x, 0), nr);
+    Rmed = t.Rmed[r];
+    idm = t.invdiffRmed[r];
+    Rinf = t.Rinf[r];
+    isurf = t.invSurf[r];
+    // consume the loads here, so that no wait for them lands in the loop
+    asm volatile("" ::"v"(Rmed), "v"(idm), "v"(Rinf), "v"(isurf));
+  }
+  // ring m - 2 .. m + 61 can be read: reload when ring m would leave that
+  __device__ __forceinline__ int at(const fg_dev_tables &t, int m, int nr) {
+    if (m - r0 >= 64) load(t, m - 2, nr);
+    return m - r0;
+  }
+};
+
 __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   const int nr = a.nr, ns = a.ns;
   // blocks [0, NRAD): one ring mean each, issued first so that these
@@ -379,8 +413,10 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     return;
   }
   const int bid = blockIdx.x - nr;
-  const int j = (bid % a.nstrips) * TRAD_TPB + threadIdx.x;
-  if (j >= ns) return;
+  // lanes past the last sector march on column ns - 1 (every lane holds
+  // its ring table entry) and store the same values as the lane that owns
+  // that column
+  const int j = min((bid % a.nstrips) * TRAD_TPB + (int)threadIdx.x, ns - 1);
   const int jp = (j + 1) % ns;
   const int b0 = (bid / a.nstrips) * TRAD_BAND;
   const int b1 = min(b0 + TRAD_BAND, nr);
@@ -390,26 +426,33 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   // rolling window: index 0 = ring m-2, 1 = m-1, 2 = m
   double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
   double sg[3] = {}, dqs[2] = {}, vr[3] = {};
-  int m0 = max(b0 - 2, 0);
-  // ring m's inputs are loaded one iteration ahead
-  double n_s = 0.0, n_v0 = 0.0, n_v1 = 0.0, n_vtp = 0.0, n_vt = 0.0;
-  if (m0 < nr) {
+  const int m0 = max(b0 - 2, 0);
+  ring_lanes rt;
+  rt.load(t, m0 - 2, nr);
+  // ring m's inputs are loaded one iteration ahead (ring min(m + 1, nr - 1):
+  // the loads are unconditional, so that every iteration of a loop issues
+  // the same memory operations and the waits for them can count past the
+  // stores); vrad of ring m + 1 is carried over as the next ring's own
+  double n_s, n_v0, n_v1, n_vtp, n_vt;
+  {
     const long l = (long)m0 * ns + j;
     n_s = a.dens[l];
     n_v0 = a.vr[l];
     n_v1 = a.vr[l + ns];
     n_vtp = a.vt[(long)m0 * ns + jp];
     n_vt = a.vt[l];
+    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
   }
-  for (int m = m0; m <= b1 + 1; m++) {
+  auto ring = [&](int m, auto store) {
+    const int r = rt.at(t, m, nr);
     const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
-    if (m + 1 < nr) {
-      const long l = (long)(m + 1) * ns + j;
-      n_s = a.dens[l];
-      n_v0 = a.vr[l];
-      n_v1 = a.vr[l + ns];
-      n_vtp = a.vt[(long)(m + 1) * ns + jp];
-      n_vt = a.vt[l];
+    {
+      const long l = (long)min(m + 1, nr - 1) * ns;
+      n_s = a.dens[l + j];
+      n_v0 = c_v1;
+      n_v1 = a.vr[l + ns + j];
+      n_vtp = a.vt[l + jp];
+      n_vt = a.vt[l + j];
     }
     // shift the window
     for (int q = 0; q < NQ; q++) {
@@ -422,10 +465,11 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     sg[1] = sg[2];
     vr[0] = vr[1];
     vr[1] = vr[2];
+    // ring tables: Rmed[m], Rmed[m-1], Rmed[m-2] at lanes r, r-1, r-2
+    const double Rm = lane_get(rt.Rmed, r);
     if (m < nr) {
       const double s = c_s;
       const double v0 = c_v0, v1 = c_v1;
-      const double Rm = t.Rmed[m];
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
       Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
@@ -441,41 +485,51 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     }
     // slope of ring m-1 (zero on rings 0 and NRAD-1)
     const int k = m - 1;
-    if (k >= 0) {
-      for (int q = 0; q < NQ; q++) {
-        dq[q][0] = dq[q][1];
-        dq[q][1] = (k == 0 || k >= nr - 1) ? 0.0 : vl_rad_slope(W[q][0], W[q][1], W[q][2], t.invdiffRmed[k], t.invdiffRmed[k + 1]);
-      }
-      dqs[0] = dqs[1];
-      dqs[1] = (k == 0 || k >= nr - 1) ? 0.0 : vl_rad_slope(sg[0], sg[1], sg[2], t.invdiffRmed[k], t.invdiffRmed[k + 1]);
-      // face k: upwind values (StarRadKernel2); faces 0 and NRAD are zero
-      double DS = 0.0;
-      double QS[NQ];
-      if (k > 0 && k < nr) {
-        const double vk = vr[1];
-        DS = vl_rad_star(vk, dt, sg[0], sg[1], dqs[0], dqs[1], t.Rmed[k - 1], t.Rmed[k], t.Rmed[k + 1], k);
-        for (int q = 0; q < NQ; q++)
-          QS[q] = vl_rad_star(vk, dt, W[q][0], W[q][1], dq[q][0], dq[q][1], t.Rmed[k - 1], t.Rmed[k], t.Rmed[k + 1], k);
-      } else {
-        for (int q = 0; q < NQ; q++) QS[q] = 0.0;
-      }
-      const double vface = (k < nr) ? vr[1] : 0.0;
-      for (int q = 0; q < NQ; q++) {
-        F[q][0] = F[q][1];
-        F[q][1] = cflux * t.Rinf[k] * QS[q] * DS * vface;
-      }
-      // ring k-1 is complete: Q += (F(k-1) - F(k)) / Surf
+    if (!decltype(store)::value && k < 0) return;  // storing steps have k >= 1
+    const bool flat = (k == 0 || k >= nr - 1);
+    const double idm0 = lane_get(rt.idm, r - 1), idm1 = lane_get(rt.idm, r);
+    for (int q = 0; q < NQ; q++) {
+      dq[q][0] = dq[q][1];
+      dq[q][1] = flat ? 0.0 : vl_rad_slope(W[q][0], W[q][1], W[q][2], idm0, idm1);
+    }
+    dqs[0] = dqs[1];
+    dqs[1] = flat ? 0.0 : vl_rad_slope(sg[0], sg[1], sg[2], idm0, idm1);
+    // face k: upwind values (StarRadKernel2); faces 0 and NRAD are zero
+    double DS = 0.0;
+    double QS[NQ];
+    if (k > 0 && k < nr) {
+      const double vk = vr[1];
+      const double Rkm = lane_get(rt.Rmed, r - 2), Rk = lane_get(rt.Rmed, r - 1);
+      DS = vl_rad_star(vk, dt, sg[0], sg[1], dqs[0], dqs[1], Rkm, Rk, Rm, k);
+      for (int q = 0; q < NQ; q++) QS[q] = vl_rad_star(vk, dt, W[q][0], W[q][1], dq[q][0], dq[q][1], Rkm, Rk, Rm, k);
+    } else {
+      for (int q = 0; q < NQ; q++) QS[q] = 0.0;
+    }
+    const double vface = (k < nr) ? vr[1] : 0.0;
+    const double Rinf = lane_get(rt.Rinf, r - 1);
+    for (int q = 0; q < NQ; q++) {
+      F[q][0] = F[q][1];
+      F[q][1] = cflux * Rinf * QS[q] * DS * vface;
+    }
+    // ring k-1 is complete: Q += (F(k-1) - F(k)) / Surf
+    if (decltype(store)::value) {
       const int o = k - 1;
-      if (o >= b0 && o < b1) {
-        const long lo = (long)o * ns + j;
-        for (int q = 0; q < NQ; q++) {
-          double varq = F[q][0];
-          varq -= F[q][1];
-          a.out[q][lo] = Q[q][0] + varq * t.invSurf[o];
-        }
+      const long lo = (long)o * ns + j;
+      const double isurf = lane_get(rt.isurf, r - 2);
+      for (int q = 0; q < NQ; q++) {
+        double varq = F[q][0];
+        varq -= F[q][1];
+        a.out[q][lo] = Q[q][0] + varq * isurf;
       }
     }
-  }
+  };
+  // rings m0 .. b0 + 1 fill the window; from m = b0 + 2 on ring m - 2 of
+  // the band is complete after every step
+  int m = m0;
+  for (; m < b0 + 2; m++) ring(m, std::false_type());
+  // the storing loop starts with nothing in flight from the filling one
+  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+  for (; m <= b1 + 1; m++) ring(m, std::true_type());
 }
 
 // ------------------------------------------------------------ k_taz ----
@@ -1364,8 +1418,8 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     return;
   }
   const int bid = blk - nr;
-  const int j = (bid % a.nstrips) * TRAD_TPB + threadIdx.x;
-  if (j >= ns) return;
+  // lanes past the last sector: as in k_trad
+  const int j = min((bid % a.nstrips) * TRAD_TPB + (int)threadIdx.x, ns - 1);
   const int jp = (j + 1) % ns;
   const int b0 = (bid / a.nstrips) * a.band;
   const int b1 = min(b0 + a.band, nr);
@@ -1376,26 +1430,31 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   double Q[NQ][3] = {};
   float W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
   float sg[3] = {}, dqs[2] = {}, vr[3] = {};
-  int m0 = max(b0 - 2, 0);
-  // ring m's inputs are loaded one iteration ahead
-  double n_s = 0.0, n_v0 = 0.0, n_v1 = 0.0, n_vtp = 0.0, n_vt = 0.0;
-  if (m0 < nr) {
+  const int m0 = max(b0 - 2, 0);
+  ring_lanes rt;
+  rt.load(t, m0 - 2, nr);
+  // ring m's inputs are loaded one iteration ahead, unconditionally (as
+  // in k_trad)
+  double n_s, n_v0, n_v1, n_vtp, n_vt;
+  {
     const long l = (long)m0 * ns + j;
     n_s = a.dens[l];
     n_v0 = a.vr[l];
     n_v1 = a.vr[l + ns];
     n_vtp = a.vt[(long)m0 * ns + jp];
     n_vt = a.vt[l];
+    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
   }
-  for (int m = m0; m <= b1 + 1; m++) {
+  auto ring = [&](int m, auto store) {
+    const int r = rt.at(t, m, nr);
     const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
-    if (m + 1 < nr) {
-      const long l = (long)(m + 1) * ns + j;
-      n_s = a.dens[l];
-      n_v0 = a.vr[l];
-      n_v1 = a.vr[l + ns];
-      n_vtp = a.vt[(long)(m + 1) * ns + jp];
-      n_vt = a.vt[l];
+    {
+      const long l = (long)min(m + 1, nr - 1) * ns;
+      n_s = a.dens[l + j];
+      n_v0 = c_v1;
+      n_v1 = a.vr[l + ns + j];
+      n_vtp = a.vt[l + jp];
+      n_vt = a.vt[l + j];
     }
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
@@ -1407,10 +1466,10 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     sg[1] = sg[2];
     vr[0] = vr[1];
     vr[1] = vr[2];
+    const double Rm = lane_get(rt.Rmed, r);
     if (m < nr) {
       const double s = c_s;
       const double v0 = c_v0, v1 = c_v1;
-      const double Rm = t.Rmed[m];
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
       Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
@@ -1430,38 +1489,42 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
       vr[2] = 0.0f;
     }
     const int k = m - 1;
-    if (k >= 0) {
-      const bool flat = (k == 0 || k >= nr - 1);
-      const float idm0 = flat ? 0.f : (float)t.invdiffRmed[k], idm1 = flat ? 0.f : (float)t.invdiffRmed[k + 1];
-      for (int q = 0; q < NQ; q++) {
-        dq[q][0] = dq[q][1];
-        dq[q][1] = flat ? 0.0f : vl_rad_slope_f(W[q][0], W[q][1], W[q][2], idm0, idm1);
-      }
-      dqs[0] = dqs[1];
-      dqs[1] = flat ? 0.0f : vl_rad_slope_f(sg[0], sg[1], sg[2], idm0, idm1);
-      float DS = 0.0f, QS[NQ];
-      if (k > 0 && k < nr) {
-        const float vk = vr[1];
-        const float dm = (float)(t.Rmed[k] - t.Rmed[k - 1]), dp = (float)(t.Rmed[k + 1] - t.Rmed[k]);
-        DS = vl_rad_star_f(vk, dtf, sg[0], sg[1], dqs[0], dqs[1], dm, dp);
-        for (int q = 0; q < NQ; q++) QS[q] = vl_rad_star_f(vk, dtf, W[q][0], W[q][1], dq[q][0], dq[q][1], dm, dp);
-      } else {
-        for (int q = 0; q < NQ; q++) QS[q] = 0.0f;
-      }
-      const float vface = (k < nr) ? vr[1] : 0.0f;
-      const float cf = cflux * (float)t.Rinf[k] * DS * vface;
-      for (int q = 0; q < NQ; q++) {
-        F[q][0] = F[q][1];
-        F[q][1] = cf * QS[q];
-      }
-      const int o = k - 1;
-      if (o >= b0 && o < b1) {
-        const long lo = (long)o * ns + j;
-        const float isurf = (float)t.invSurf[o];
-        for (int q = 0; q < NQ; q++) a.out[q][lo] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
-      }
+    if (!decltype(store)::value && k < 0) return;  // storing steps have k >= 1
+    const bool flat = (k == 0 || k >= nr - 1);
+    const float idm0 = flat ? 0.f : (float)lane_get(rt.idm, r - 1), idm1 = flat ? 0.f : (float)lane_get(rt.idm, r);
+    for (int q = 0; q < NQ; q++) {
+      dq[q][0] = dq[q][1];
+      dq[q][1] = flat ? 0.0f : vl_rad_slope_f(W[q][0], W[q][1], W[q][2], idm0, idm1);
     }
-  }
+    dqs[0] = dqs[1];
+    dqs[1] = flat ? 0.0f : vl_rad_slope_f(sg[0], sg[1], sg[2], idm0, idm1);
+    float DS = 0.0f, QS[NQ];
+    if (k > 0 && k < nr) {
+      const float vk = vr[1];
+      const double Rk = lane_get(rt.Rmed, r - 1);
+      const float dm = (float)(Rk - lane_get(rt.Rmed, r - 2)), dp = (float)(Rm - Rk);
+      DS = vl_rad_star_f(vk, dtf, sg[0], sg[1], dqs[0], dqs[1], dm, dp);
+      for (int q = 0; q < NQ; q++) QS[q] = vl_rad_star_f(vk, dtf, W[q][0], W[q][1], dq[q][0], dq[q][1], dm, dp);
+    } else {
+      for (int q = 0; q < NQ; q++) QS[q] = 0.0f;
+    }
+    const float vface = (k < nr) ? vr[1] : 0.0f;
+    const float cf = cflux * (float)lane_get(rt.Rinf, r - 1) * DS * vface;
+    for (int q = 0; q < NQ; q++) {
+      F[q][0] = F[q][1];
+      F[q][1] = cf * QS[q];
+    }
+    if (decltype(store)::value) {
+      const int o = k - 1;
+      const long lo = (long)o * ns + j;
+      const float isurf = (float)lane_get(rt.isurf, r - 2);
+      for (int q = 0; q < NQ; q++) a.out[q][lo] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
+    }
+  };
+  int m = m0;
+  for (; m < b0 + 2; m++) ring(m, std::false_type());
+  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+  for (; m <= b1 + 1; m++) ring(m, std::true_type());
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
