@@ -97,6 +97,25 @@ __device__ __forceinline__ void ev_consts(const fg_dev_tables &t, const double *
   vrad0 = -3.0 * nu[i] / Rm * t.vrad0_slope;
 }
 
+// per-ring tables the source stages read, staged per tile row in LDS
+enum { RT_MED, RT_IMED, RT_IDMED, RT_INF, RT_IINF, RT_IDXT, RT_SUPP, RT_IDSUP, RT_SUP, RT_NU, RT_CS, RT_SMED, RT_EVZ, RT_N };
+__device__ __forceinline__ const double *ring_table(const fg_dev_tables &t, const double *nu, int x) {
+  switch (x) {
+    case RT_MED: return t.Rmed;
+    case RT_IMED: return t.invRmed;
+    case RT_IDMED: return t.invdiffRmed;
+    case RT_INF: return t.Rinf;
+    case RT_IINF: return t.invRinf;
+    case RT_IDXT: return t.invdxtheta;
+    case RT_SUPP: return t.supp_torque;
+    case RT_IDSUP: return t.invdiffRsup;
+    case RT_SUP: return t.Rsup;
+    case RT_NU: return nu;
+    case RT_CS: return t.cs_iso;
+    default: return t.SigmaMed;
+  }
+}
+
 template <int KTH, int KTW>
 __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   constexpr int KHR = KTH + HALO_R + HALO_B, KHC = KTW + 2 * HALO_C;
@@ -118,34 +137,58 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
   // all of a thread's loads are issued before any of them is used
   double ls[KPT], lvr[KPT], lvt[KPT];
-  int lj[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
     const int q = tid + k * SRC_TPB;
     ls[k] = lvr[k] = lvt[k] = 0.0;
-    lj[k] = 0;
     if (q < KHR * KHC) {
       const int r = q / KHC, cc = q - r * KHC, i = GI(r);
       if (i >= 0 && i <= nr) {
         const int j = GJ(cc);
         const long l = (long)i * ns + j;
-        lj[k] = j;
         ls[k] = a.dens[l];
         lvr[k] = a.vr[l];
         lvt[k] = a.vt[l];
       }
     }
   }
-  // damping constants of the tile's rings, once per ring while the loads
-  // above are in flight (Kernels.cu:978-1004)
+  // The per-ring tables of the tile's rows (rings clamped into the table;
+  // rows outside the disk are never read) and the potential's cos/sin of
+  // its columns, staged in LDS while the loads above are in flight: every
+  // stage below reads them from there.  Read from memory inside a stage,
+  // each would be a dependent round trip.  Each wave loads every fourth
+  // table, lane r row r.
+  __shared__ double rt[RT_N][KHR];
+  __shared__ double ct[2][KHC];
+  {
+    const int w = tid >> 6, ln = tid & 63;
+    if (ln < KHR) {
+      const int i = min(max(GI(ln), 0), nr - 1);
+      // wave w: tables w, w + 4, w + 8 and (wave 0) the ev_zone flags;
+      // all loads issued before the first LDS write
+      const double v0 = ring_table(t, a.nu, w)[i], v1 = ring_table(t, a.nu, w + 4)[i];
+      const double v2 = ring_table(t, a.nu, w + 8)[i];
+      const int ez = t.ev_zone[i];
+      rt[w][ln] = v0;
+      rt[w + 4][ln] = v1;
+      rt[w + 8][ln] = v2;
+      if (w == 0) rt[RT_EVZ][ln] = (double)ez;
+    }
+    for (int c = tid; c < KHC; c += SRC_TPB) {
+      const int j = GJ(c);
+      ct[0][c] = t.cos_pot[j];
+      ct[1][c] = t.sin_pot[j];
+    }
+  }
+  // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004)
   __shared__ double evc[KHR][3];
   if (a.evanescent) {
     for (int r = tid; r < KHR; r += SRC_TPB) {
       const int i = GI(r);
       if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
     }
-    __syncthreads();
   }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
     const int q = tid + k * SRC_TPB;
@@ -154,20 +197,19 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     int i = GI(r);
     double s = 0.0, vr = 0.0, vt = 0.0, p = 0.0, phi = 0.0;
     if (i >= 0 && i <= nr) {
-      const int j = lj[k];
       s = ls[k];
       vr = lvr[k];
       vt = lvt[k];
       if (i < nr) {
-        if (a.corotating) vt = vt - a.domega * t.Rmed[i];
-        if (a.evanescent && t.ev_zone[i]) {
+        if (a.corotating) vt = vt - a.domega * rt[RT_MED][r];
+        if (a.evanescent && rt[RT_EVZ][r] != 0.0) {
           vr = damp_toward(vr, evc[r][0], evc[r][1]);
           vt = damp_toward(vt, evc[r][0], evc[r][2]);
-          s = damp_toward(s, evc[r][0], t.SigmaMed[i]);
+          s = damp_toward(s, evc[r][0], rt[RT_SMED][r]);
         }
-        const double cs = t.cs_iso[i];
+        const double cs = rt[RT_CS][r];
         p = s * cs * cs;
-        phi = cell_potential(a.P, t.Rmed[i], t.invRmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
+        phi = cell_potential(a.P, rt[RT_MED][r], rt[RT_IMED][r], ct[0][cc], ct[1][cc], a.indirect);
       }
     }
     sS[r][cc] = s;
@@ -190,20 +232,20 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       double vri = 0.0, vti = 0.0;
       if (r >= 1 && cc >= 1 && cc < KHC - 1 && i >= 0 && i < nr) {
         if (i > 0) {
-          double gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0 / (sS[r][cc] + sS[r - 1][cc]) * t.invdiffRmed[i];
-          double gradphi = (sB[r][cc] - sB[r - 1][cc]) * t.invdiffRmed[i];
+          double gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0 / (sS[r][cc] + sS[r - 1][cc]) * rt[RT_IDMED][r];
+          double gradphi = (sB[r][cc] - sB[r - 1][cc]) * rt[RT_IDMED][r];
           double vt2 = sC[r][cc] + sC[r][cc + 1] + sC[r - 1][cc] + sC[r - 1][cc + 1];
-          vt2 = vt2 / 4.0 + a.omega_frame * t.Rinf[i];
+          vt2 = vt2 / 4.0 + a.omega_frame * rt[RT_INF][r];
           vt2 = vt2 * vt2;
           double vr_src = -gradp - gradphi;
-          double vradint2 = vr_src + vt2 * t.invRinf[i];
+          double vradint2 = vr_src + vt2 * rt[RT_IINF][r];
           vri = sD[r][cc] + dt * vradint2;
         }
-        const double invdxtheta = t.invdxtheta[i];
+        const double invdxtheta = rt[RT_IDXT][r];
         double gradp = (sA[r][cc] - sA[r][cc - 1]) * 2.0 / (sS[r][cc] + sS[r][cc - 1]) * invdxtheta;
         double gradphi = (sB[r][cc] - sB[r][cc - 1]) * invdxtheta;
         double v = sC[r][cc] - dt * (gradp + gradphi);
-        v += dt * t.supp_torque[i];
+        v += dt * rt[RT_SUPP][r];
         vti = v;
       }
       regA[k] = vri;
@@ -239,16 +281,17 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     int i = GI(r);
     if (r < 1 || r >= KHR - 1 || cc < 2 || cc >= KHC - 2 || i < 0 || i >= nr) continue;
     const double *VrR = &sA[r][0];
-    double drr = (sA[r + 1][cc] - VrR[cc]) * t.invdiffRsup[i];
-    double dpp = (sB[r][cc + 1] - sB[r][cc]) * invdphi * t.invRmed[i] + 0.5 * (sA[r + 1][cc] + VrR[cc]) * t.invRmed[i];
-    double dv = (sA[r + 1][cc] * t.Rsup[i] - VrR[cc] * t.Rinf[i]) * t.invdiffRsup[i] * t.invRmed[i];
-    dv += (sB[r][cc + 1] - sB[r][cc]) * invdphi * t.invRmed[i];
-    const double nu = a.nu[i];
+    const double invRm = rt[RT_IMED][r], idsup = rt[RT_IDSUP][r];
+    double drr = (sA[r + 1][cc] - VrR[cc]) * idsup;
+    double dpp = (sB[r][cc + 1] - sB[r][cc]) * invdphi * invRm + 0.5 * (sA[r + 1][cc] + VrR[cc]) * invRm;
+    double dv = (sA[r + 1][cc] * rt[RT_SUP][r] - VrR[cc] * rt[RT_INF][r]) * idsup * invRm;
+    dv += (sB[r][cc + 1] - sB[r][cc]) * invdphi * invRm;
+    const double nu = rt[RT_NU][r];
     sC[r][cc] = 2.0 * sS[r][cc] * nu * (drr - onethird * dv);
     sD[r][cc] = 2.0 * sS[r][cc] * nu * (dpp - onethird * dv);
     if (i > 0) {
-      double drp = 0.5 * (t.Rinf[i] * (sB[r][cc] * t.invRmed[i] - sB[r - 1][cc] * t.invRmed[i - 1]) * t.invdiffRmed[i] +
-                          (VrR[cc] - VrR[cc - 1]) * invdphi * t.invRinf[i]);
+      double drp = 0.5 * (rt[RT_INF][r] * (sB[r][cc] * invRm - sB[r - 1][cc] * rt[RT_IMED][r - 1]) * rt[RT_IDMED][r] +
+                          (VrR[cc] - VrR[cc - 1]) * invdphi * rt[RT_IINF][r]);
       sE[r][cc] = 2.0 * 0.25 * (sS[r][cc] + sS[r - 1][cc] + sS[r][cc - 1] + sS[r - 1][cc - 1]) * nu * drp;
     }
   }
@@ -263,12 +306,12 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     int i = GI(r);
     if (r < 2 || r >= KHR - 1 || cc < 3 || cc >= KHC - 3 || i < 1 || i >= nr) continue;
     if (i < nr - 1 && r < KHR - 2)
-      sB[r][cc] += dt * t.invRmed[i] *
-                   ((t.Rsup[i] * sE[r + 1][cc] - t.Rinf[i] * sE[r][cc]) * t.invdiffRsup[i] +
+      sB[r][cc] += dt * rt[RT_IMED][r] *
+                   ((rt[RT_SUP][r] * sE[r + 1][cc] - rt[RT_INF][r] * sE[r][cc]) * rt[RT_IDSUP][r] +
                     (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5 * (sE[r][cc] + sE[r + 1][cc])) /
                    (0.5 * (sS[r][cc] + sS[r][cc - 1]));
-    sA[r][cc] += dt * t.invRinf[i] *
-                 ((t.Rmed[i] * sC[r][cc] - t.Rmed[i - 1] * sC[r - 1][cc]) * t.invdiffRmed[i] +
+    sA[r][cc] += dt * rt[RT_IINF][r] *
+                 ((rt[RT_MED][r] * sC[r][cc] - rt[RT_MED][r - 1] * sC[r - 1][cc]) * rt[RT_IDMED][r] +
                   (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5 * (sD[r][cc] + sD[r - 1][cc])) /
                  (0.5 * (sS[r][cc] + sS[r - 1][cc]));
   }
@@ -306,17 +349,17 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     if (i > 0) {
       double d = sS[r][cc] + sS[r - 1][cc];
       double densint = sC[r][cc] - sC[r - 1][cc];
-      vrnew = sA[r][cc] - dt * 2.0 / d * densint * t.invdiffRmed[i];
+      vrnew = sA[r][cc] - dt * 2.0 / d * densint * rt[RT_IDMED][r];
     }
-    const double invdxtheta = t.invdxtheta[i];
+    const double invdxtheta = rt[RT_IDXT][r];
     double dens2 = sS[r][cc] + sS[r][cc - 1];
     double tempint = (sD[r][cc] - sD[r][cc - 1]);
     double vtnew = sB[r][cc] - dt * 2.0 / dens2 * tempint * invdxtheta;
     double s = sS[r][cc];
-    if (a.evanescent && t.ev_zone[i]) {
+    if (a.evanescent && rt[RT_EVZ][r] != 0.0) {
       vrnew = damp_toward(vrnew, evc[r][0], evc[r][1]);
       vtnew = damp_toward(vtnew, evc[r][0], evc[r][2]);
-      s = damp_toward(s, evc[r][0], t.SigmaMed[i]);
+      s = damp_toward(s, evc[r][0], rt[RT_SMED][r]);
     }
     a.odens[l] = s;
     a.ovr[l] = vrnew;
@@ -1116,16 +1159,16 @@ __device__ __forceinline__ float cell_potential_planets_f(const fg_planets_arg &
 }
 
 // CorrectVtheta and the first damping pass of one cell's state (FP64);
-// ev = the ring's damping constants (lambda, vrad0, vtheta0)
-__device__ __forceinline__ void src32_adjust(const src32_args &a, int i, const double *ev, double &s, double &vr,
-                                             double &vt) {
-  const fg_dev_tables &t = a.t;
+// rt = the ring's tables (RT_* layout, one value per ring), ev = its
+// damping constants (lambda, vrad0, vtheta0)
+__device__ __forceinline__ void src32_adjust(const src32_args &a, int i, double rmed, double evz, double smed,
+                                             const double *ev, double &s, double &vr, double &vt) {
   if (i < a.nr) {
-    if (a.corotating) vt = vt - a.domega * t.Rmed[i];
-    if (a.evanescent && t.ev_zone[i]) {
+    if (a.corotating) vt = vt - a.domega * rmed;
+    if (a.evanescent && evz != 0.0) {
       vr = damp_toward(vr, ev[0], ev[1]);
       vt = damp_toward(vt, ev[0], ev[2]);
-      s = damp_toward(s, ev[0], t.SigmaMed[i]);
+      s = damp_toward(s, ev[0], smed);
     }
   }
 }
@@ -1147,9 +1190,8 @@ __device__ __forceinline__ void own_load(const src32_args &a, int i0, int j0, in
   }
 }
 
-// ROWC: the damping constants of the tile's rings are computed once per
-// ring into LDS; otherwise once per cell
-template <bool ROWC>
+// the damping constants of the tile's rings are computed once per ring
+// into LDS
 __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, int th, float *lds) {
   const int hr = th + HALO_R + HALO_B;
   float(*sS)[HC] = (float(*)[HC])(lds);           // Sigma after the first damping
@@ -1169,13 +1211,11 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   const float dtf = (float)dt;
 #define GI(r) (i0 - HALO_R + (r))
 #define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
-#define VREF(i) (((i) >= 0 && (i) < nr) ? a.vref[i] : 0.0)
-  // the damping constants of ring i of row r
-  auto ring_ev = [&](int r, int i, double *own) -> const double * {
-    if (ROWC) return evc[r];
-    if (a.evanescent && i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, own[0], own[1], own[2]);
-    return own;
-  };
+  // per-ring tables of the tile's rows as in k_src, plus the reference
+  // vtheta (0 outside the disk), and the potential's cos/sin per column
+  constexpr int RT_VREF = RT_N, MAXR = SRC32_MAX_TH + HALO_R + HALO_B;
+  __shared__ double rt[RT_N + 1][MAXR];
+  __shared__ double ct[2][HC];
 
   // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
   // all of a thread's loads are issued before any of them is used
@@ -1195,15 +1235,37 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
         }
       }
     }
-  // damping constants of the tile's rings, once per ring while the loads
-  // above are in flight (Kernels.cu:978-1004)
-  if (ROWC && a.evanescent) {
-    for (int r = tid; r < hr; r += SRC_TPB) {
-      const int i = GI(r);
-      if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
+    // tables staged while the loads above are in flight: wave w loads
+    // tables w, w + 4, w + 8 (wave 0 also the ev_zone flags, wave 1 the
+    // reference vtheta) of rows ln and ln + 64
+    {
+      const int w = tid >> 6, ln = tid & 63;
+      for (int r = ln; r < hr; r += 64) {
+        const int gi = GI(r), i = min(max(gi, 0), nr - 1);
+        const double v0 = ring_table(t, a.nu, w)[i], v1 = ring_table(t, a.nu, w + 4)[i];
+        const double v2 = ring_table(t, a.nu, w + 8)[i];
+        const int ez = t.ev_zone[i];
+        const double vref = (gi >= 0 && gi < nr) ? a.vref[gi] : 0.0;
+        rt[w][r] = v0;
+        rt[w + 4][r] = v1;
+        rt[w + 8][r] = v2;
+        if (w == 0) rt[RT_EVZ][r] = (double)ez;
+        if (w == 1) rt[RT_VREF][r] = vref;
+      }
+      for (int c = tid; c < HC; c += SRC_TPB) {
+        const int j = GJ(c);
+        ct[0][c] = t.cos_pot[j];
+        ct[1][c] = t.sin_pot[j];
+      }
+    }
+    // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004)
+    if (a.evanescent) {
+      for (int r = tid; r < hr; r += SRC_TPB) {
+        const int i = GI(r);
+        if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
+      }
     }
     __syncthreads();
-  }
 #pragma unroll
   for (int k = 0; k < SRC32_PER_THREAD; k++) {
     const int q = tid + k * SRC_TPB;
@@ -1212,17 +1274,15 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     int i = GI(r);
     float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
     if (i >= 0 && i <= nr) {
-      int j = GJ(cc);
       double sd = ls[k], vrd = lvr[k], vtd = lvt[k];
-      double own[3];
-      src32_adjust(a, i, ring_ev(r, i, own), sd, vrd, vtd);
+      src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], evc[r], sd, vrd, vtd);
       s = (float)sd;
       vr = (float)vrd;
-      vt = (float)(vtd - VREF(i));
+      vt = (float)(vtd - rt[RT_VREF][r]);
       if (i < nr) {
-        const double cs = t.cs_iso[i];
+        const double cs = rt[RT_CS][r];
         p = (float)(sd * cs * cs);
-        phi = cell_potential_planets_f(a.P, t.Rmed[i], t.cos_pot[j], t.sin_pot[j], a.indirect);
+        phi = cell_potential_planets_f(a.P, rt[RT_MED][r], ct[0][cc], ct[1][cc], a.indirect);
       }
     }
     sS[r][cc] = s;
@@ -1247,21 +1307,21 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
       float vri = 0.f, vti = 0.f, dvr = 0.f, dvt = 0.f;
       if (r >= 1 && cc >= 1 && cc < HC - 1 && i >= 0 && i < nr) {
         if (i > 0) {
-          const float idr = (float)t.invdiffRmed[i];
+          const float idr = (float)rt[RT_IDMED][r];
           float gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0f * rcp_f(sS[r][cc] + sS[r - 1][cc]) * idr;
           float gradphi = (sB[r][cc] - sB[r - 1][cc]) * idr;
           // centrifugal minus star gravity in FP64: vbar = ring part + offset
-          double vbar = 0.5 * (VREF(i) + VREF(i - 1)) + a.omega_frame * t.Rinf[i];
+          double vbar = 0.5 * (rt[RT_VREF][r] + rt[RT_VREF][r - 1]) + a.omega_frame * rt[RT_INF][r];
           vbar += (double)((sC[r][cc] + sC[r][cc + 1] + sC[r - 1][cc] + sC[r - 1][cc + 1]) * 0.25f);
-          double bal = vbar * vbar * t.invRinf[i];
-          if (star) bal -= (t.invRmed[i - 1] - t.invRmed[i]) * t.invdiffRmed[i];
+          double bal = vbar * vbar * rt[RT_IINF][r];
+          if (star) bal -= (rt[RT_IMED][r - 1] - rt[RT_IMED][r]) * rt[RT_IDMED][r];
           dvr = dtf * (-gradp - gradphi + (float)bal);
           vri = sD[r][cc] + dvr;
         }
-        const float invdxtheta = (float)t.invdxtheta[i];
+        const float invdxtheta = (float)rt[RT_IDXT][r];
         float gradp = (sA[r][cc] - sA[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * invdxtheta;
         float gradphi = (sB[r][cc] - sB[r][cc - 1]) * invdxtheta;
-        dvt = dtf * ((float)t.supp_torque[i] - (gradp + gradphi));
+        dvt = dtf * ((float)rt[RT_SUPP][r] - (gradp + gradphi));
         vti = sC[r][cc] + dvt;
       }
       regA[k] = vri;
@@ -1294,23 +1354,23 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     if (r < 1 || r >= hr - 1 || cc < 2 || cc >= HC - 2 || i < 0 || i >= nr) continue;
-    const float iRm = (float)t.invRmed[i], idRs = (float)t.invdiffRsup[i];
+    const float iRm = (float)rt[RT_IMED][r], idRs = (float)rt[RT_IDSUP][r];
     const float vr0 = sA[r][cc], vr1 = sA[r + 1][cc];
     const float dvt = sB[r][cc + 1] - sB[r][cc];
     float drr = (vr1 - vr0) * idRs;
     float dpp = dvt * invdphi * iRm + 0.5f * (vr1 + vr0) * iRm;
-    float dv = (vr1 * (float)t.Rsup[i] - vr0 * (float)t.Rinf[i]) * idRs * iRm;
+    float dv = (vr1 * (float)rt[RT_SUP][r] - vr0 * (float)rt[RT_INF][r]) * idRs * iRm;
     dv += dvt * invdphi * iRm;
-    const float nu2s = 2.0f * sS[r][cc] * (float)a.nu[i];
+    const float nu2s = 2.0f * sS[r][cc] * (float)rt[RT_NU][r];
     sC[r][cc] = nu2s * (drr - onethird * dv);
     sD[r][cc] = nu2s * (dpp - onethird * dv);
     if (i > 0) {
       // d(vtheta/r)/dr with the reference part of vtheta differenced in FP64
-      const double dref = (VREF(i) * t.invRmed[i] - VREF(i - 1) * t.invRmed[i - 1]);
-      const float dom = (float)dref + (sB[r][cc] * iRm - sB[r - 1][cc] * (float)t.invRmed[i - 1]);
-      float drp = 0.5f * ((float)t.Rinf[i] * dom * (float)t.invdiffRmed[i] +
-                          (sA[r][cc] - sA[r][cc - 1]) * invdphi * (float)t.invRinf[i]);
-      sE[r][cc] = 0.5f * (sS[r][cc] + sS[r - 1][cc] + sS[r][cc - 1] + sS[r - 1][cc - 1]) * (float)a.nu[i] * drp;
+      const double dref = (rt[RT_VREF][r] * rt[RT_IMED][r] - rt[RT_VREF][r - 1] * rt[RT_IMED][r - 1]);
+      const float dom = (float)dref + (sB[r][cc] * iRm - sB[r - 1][cc] * (float)rt[RT_IMED][r - 1]);
+      float drp = 0.5f * ((float)rt[RT_INF][r] * dom * (float)rt[RT_IDMED][r] +
+                          (sA[r][cc] - sA[r][cc - 1]) * invdphi * (float)rt[RT_IINF][r]);
+      sE[r][cc] = 0.5f * (sS[r][cc] + sS[r - 1][cc] + sS[r][cc - 1] + sS[r - 1][cc - 1]) * (float)rt[RT_NU][r] * drp;
     }
   }
   __syncthreads();
@@ -1323,19 +1383,19 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
     if (r < 2 || r >= hr - 1 || cc < 3 || cc >= HC - 3 || i < 1 || i >= nr) continue;
-    const float iRm = (float)t.invRmed[i];
+    const float iRm = (float)rt[RT_IMED][r];
     const int ro = r - HALO_R, co = cc - HALO_C;
     const bool own = ro >= 0 && ro < th && co >= 0 && co < TW;
     if (i < nr - 1 && r < hr - 2) {
       float d = dtf * iRm *
-                (((float)t.Rsup[i] * sE[r + 1][cc] - (float)t.Rinf[i] * sE[r][cc]) * (float)t.invdiffRsup[i] +
+                (((float)rt[RT_SUP][r] * sE[r + 1][cc] - (float)rt[RT_INF][r] * sE[r][cc]) * (float)rt[RT_IDSUP][r] +
                  (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5f * (sE[r][cc] + sE[r + 1][cc])) *
                 rcp_f(0.5f * (sS[r][cc] + sS[r][cc - 1]));
       sB[r][cc] += d;
       if (own) incT[ro][co] += d;
     }
-    float d = dtf * (float)t.invRinf[i] *
-              (((float)t.Rmed[i] * sC[r][cc] - (float)t.Rmed[i - 1] * sC[r - 1][cc]) * (float)t.invdiffRmed[i] +
+    float d = dtf * (float)rt[RT_IINF][r] *
+              (((float)rt[RT_MED][r] * sC[r][cc] - (float)rt[RT_MED][r - 1] * sC[r - 1][cc]) * (float)rt[RT_IDMED][r] +
                (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5f * (sD[r][cc] + sD[r - 1][cc])) *
               rcp_f(0.5f * (sS[r][cc] + sS[r - 1][cc]));
     sA[r][cc] += d;
@@ -1370,10 +1430,9 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     if (i > nr || j >= ns) continue;
     long l = (long)i * ns + j;
     double s = os[k], vr = ovr[k], vt = ovt[k];
-    double own[3];
-    const double *ev = ring_ev(r, i, own);
-    src32_adjust(a, i, ev, s, vr, vt);
-    if (a.opress && i < nr) a.opress[l] = s * t.cs_iso[i] * t.cs_iso[i];
+    const double *ev = evc[r];
+    src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], ev, s, vr, vt);
+    if (a.opress && i < nr) a.opress[l] = s * rt[RT_CS][r] * rt[RT_CS][r];
     if (i == nr) {  // ghost ring
       a.odens[l] = s;
       a.ovr[l] = 0.0;
@@ -1382,15 +1441,15 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     }
     double vrnew = 0.0;
     if (i > 0) {
-      float dq = (sC[r][cc] - sC[r - 1][cc]) * 2.0f * rcp_f(sS[r][cc] + sS[r - 1][cc]) * (float)t.invdiffRmed[i];
+      float dq = (sC[r][cc] - sC[r - 1][cc]) * 2.0f * rcp_f(sS[r][cc] + sS[r - 1][cc]) * (float)rt[RT_IDMED][r];
       vrnew = vr + (double)(incR[ro][co] - dtf * dq);
     }
-    float dq = (sD[r][cc] - sD[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * (float)t.invdxtheta[i];
+    float dq = (sD[r][cc] - sD[r][cc - 1]) * 2.0f * rcp_f(sS[r][cc] + sS[r][cc - 1]) * (float)rt[RT_IDXT][r];
     double vtnew = vt + (double)(incT[ro][co] - dtf * dq);
-    if (a.evanescent && t.ev_zone[i]) {
+    if (a.evanescent && rt[RT_EVZ][r] != 0.0) {
       vrnew = damp_toward(vrnew, ev[0], ev[1]);
       vtnew = damp_toward(vtnew, ev[0], ev[2]);
-      s = damp_toward(s, ev[0], t.SigmaMed[i]);
+      s = damp_toward(s, ev[0], rt[RT_SMED][r]);
     }
     a.odens[l] = s;
     a.ovr[l] = vrnew;
@@ -1398,12 +1457,11 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   }
 #undef GI
 #undef GJ
-#undef VREF
 }
 
 __global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a, int th) {
   extern __shared__ float lds_src32[];
-  src32_tile<true>(a, blockIdx.x, blockIdx.y, th, lds_src32);
+  src32_tile(a, blockIdx.x, blockIdx.y, th, lds_src32);
 }
 
 // Radial sweep in FP32 (k_trad above): Q of each ring is formed in FP64,
@@ -1897,7 +1955,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *__restr
   extern __shared__ float lds_bsrc[];
   if ((int)blockIdx.x >= n) return;
   const int4 w = uniform4(items[blockIdx.x]);
-  if (D[w.x].active) src32_tile<true>(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
+  if (D[w.x].active) src32_tile(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
