@@ -762,11 +762,19 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
       while (j >= ns) j -= ns;
     }
   }
+  // every field's loads are issued before the first is staged, so the six
+  // fields cost one memory latency, not six
+  double raw[NQ + 1][CL];
 #pragma unroll
   for (int qq = 0; qq <= NQ; qq++) {
     const double *src = qq < NQ ? a.q[qq] : a.vt;
 #pragma unroll
-    for (int m = 0; m < CL; m++) stage[l + 64 * m] = src[base + jl[m]];
+    for (int m = 0; m < CL; m++) raw[qq][m] = src[base + jl[m]];
+  }
+#pragma unroll
+  for (int qq = 0; qq <= NQ; qq++) {
+#pragma unroll
+    for (int m = 0; m < CL; m++) stage[l + 64 * m] = raw[qq][m];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < CL; c++) {
