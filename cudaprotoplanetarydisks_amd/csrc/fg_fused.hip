@@ -39,6 +39,7 @@
 #define HR (TH + HALO_R + HALO_B)
 #define HC (TW + 2 * HALO_C)
 #define SRC_TPB 256
+#define SRC64_TPB 512  // k_src: two 80 KB workgroups per CU, four waves per SIMD
 #define SRC_PER_THREAD ((HR * HC + SRC_TPB - 1) / SRC_TPB)
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
@@ -116,10 +117,10 @@ __device__ __forceinline__ const double *ring_table(const fg_dev_tables &t, cons
   }
 }
 
-template <int KTH, int KTW>
-__global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
+template <int KTH, int KTW, int KTPB>
+__global__ __launch_bounds__(KTPB) void k_src(src_args a) {
   constexpr int KHR = KTH + HALO_R + HALO_B, KHC = KTW + 2 * HALO_C;
-  constexpr int KPT = (KHR * KHC + SRC_TPB - 1) / SRC_TPB;
+  constexpr int KPT = (KHR * KHC + KTPB - 1) / KTPB;
   __shared__ double sS[KHR][KHC];   // Sigma after the first damping
   __shared__ double sA[KHR][KHC];   // pressure, then VradInt
   __shared__ double sB[KHR][KHC];   // potential, then VthetaInt
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   double ls[KPT], lvr[KPT], lvt[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
-    const int q = tid + k * SRC_TPB;
+    const int q = tid + k * KTPB;
     ls[k] = lvr[k] = lvt[k] = 0.0;
     if (q < KHR * KHC) {
       const int r = q / KHC, cc = q - r * KHC, i = GI(r);
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __shared__ double ct[2][KHC];
   {
     const int w = tid >> 6, ln = tid & 63;
-    if (ln < KHR) {
+    if (w < 4 && ln < KHR) {
       const int i = min(max(GI(ln), 0), nr - 1);
       // wave w: tables w, w + 4, w + 8 and (wave 0) the ev_zone flags;
       // all loads issued before the first LDS write
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       rt[w + 8][ln] = v2;
       if (w == 0) rt[RT_EVZ][ln] = (double)ez;
     }
-    for (int c = tid; c < KHC; c += SRC_TPB) {
+    for (int c = tid; c < KHC; c += KTPB) {
       const int j = GJ(c);
       ct[0][c] = t.cos_pot[j];
       ct[1][c] = t.sin_pot[j];
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004)
   __shared__ double evc[KHR][3];
   if (a.evanescent) {
-    for (int r = tid; r < KHR; r += SRC_TPB) {
+    for (int r = tid; r < KHR; r += KTPB) {
       const int i = GI(r);
       if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
     }
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
-    const int q = tid + k * SRC_TPB;
+    const int q = tid + k * KTPB;
     if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   {
 #pragma unroll
     for (int k = 0; k < KPT; k++) {
-      const int q = tid + k * SRC_TPB;
+      const int q = tid + k * KTPB;
       if (q >= KHR * KHC) break;
       int r = q / KHC, cc = q - r * KHC;
       int i = GI(r);
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
       regB[k] = vti;
     }
     if (a.opress) {  // pressure for the temperature output, own cells only
-      for (int q = tid; q < KTH * KTW; q += SRC_TPB) {
+      for (int q = tid; q < KTH * KTW; q += KTPB) {
         int r = q / KTW + HALO_R, cc = q % KTW + HALO_C;
         int i = GI(r), j = j0 + cc - HALO_C;
         if (i < nr && j < ns) a.opress[(long)i * ns + j] = sA[r][cc];
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KPT; k++) {
-      const int q = tid + k * SRC_TPB;
+      const int q = tid + k * KTPB;
       if (q >= KHR * KHC) break;
       int r = q / KHC, cc = q - r * KHC;
       sA[r][cc] = regA[k];  // VradInt (0 on rows 0 and NRAD, as never written)
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   const double onethird = 1.0 / 3.0;
 #pragma unroll
   for (int kq = 0; kq < KPT; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * KTPB;
     if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // ---- S3: viscous velocity update (Kernels.cu:118-150), in place ----
 #pragma unroll
   for (int kq = 0; kq < KPT; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * KTPB;
     if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
   // ---- S4: artificial pressure (Kernels.cu:409-434) on rows [3, KHR-1), cols [3, KHC-4) ----
 #pragma unroll
   for (int kq = 0; kq < KPT; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * KTPB;
     if (q >= KHR * KHC) break;
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
@@ -332,8 +333,8 @@ __global__ __launch_bounds__(SRC_TPB) void k_src(src_args a) {
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2, store ----
 #pragma unroll
-  for (int kq = 0; kq < (KTH * KTW + SRC_TPB - 1) / SRC_TPB; kq++) {
-    const int q = tid + kq * SRC_TPB;
+  for (int kq = 0; kq < (KTH * KTW + KTPB - 1) / KTPB; kq++) {
+    const int q = tid + kq * KTPB;
     if (q >= KTH * KTW) break;
     int r = q / KTW + HALO_R, cc = q % KTW + HALO_C;
     int i = GI(r), j = j0 + cc - HALO_C;
@@ -1814,7 +1815,7 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC_TPB),
                        src32_lds_bytes(th), c->stream, s3, th);
   } else {
-    hipLaunchKernelGGL((k_src<TH, TW>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC_TPB), 0, c->stream,
+    hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC64_TPB), 0, c->stream,
                        sa);
   }
   FG_LAUNCH_CHECK();
