@@ -36,11 +36,8 @@
 #define HALO_R 4  // rows above the tile (i - 4 .. )
 #define HALO_B 2  // rows below the tile ( .. i + TH + 1)
 #define HALO_C 4  // columns on each side
-#define HR (TH + HALO_R + HALO_B)
 #define HC (TW + 2 * HALO_C)
-#define SRC_TPB 256
 #define SRC64_TPB 512  // k_src: two 80 KB workgroups per CU, four waves per SIMD
-#define SRC_PER_THREAD ((HR * HC + SRC_TPB - 1) / SRC_TPB)
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
 
@@ -1135,7 +1132,8 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
 // shared memory: six (th + 6) x HC float arrays and two th x TW
 // increment arrays.
 #define SRC32_MAX_TH 32
-#define SRC32_PER_THREAD (((SRC32_MAX_TH + HALO_R + HALO_B) * HC + SRC_TPB - 1) / SRC_TPB)
+#define SRC32_TPB 512  // threads per FP32 source tile
+#define SRC32_PER_THREAD (((SRC32_MAX_TH + HALO_R + HALO_B) * HC + SRC32_TPB - 1) / SRC32_TPB)
 
 static size_t src32_lds_bytes(int th) { return (size_t)(6 * (th + HALO_R + HALO_B) * HC + 2 * th * TW) * sizeof(float); }
 
@@ -1182,7 +1180,7 @@ __device__ __forceinline__ void src32_adjust(const src32_args &a, int i, double 
   }
 }
 
-#define SRC32_OWN_MAX (SRC32_MAX_TH * TW / SRC_TPB)  // own cells per thread
+#define SRC32_OWN_MAX (SRC32_MAX_TH * TW / SRC32_TPB)  // own cells per thread
 
 // FP64 state of own cell q of the tile (0 outside the grid)
 __device__ __forceinline__ void own_load(const src32_args &a, int i0, int j0, int th, int q, double &s, double &vr,
@@ -1232,7 +1230,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     double ls[SRC32_PER_THREAD], lvr[SRC32_PER_THREAD], lvt[SRC32_PER_THREAD];
 #pragma unroll
     for (int k = 0; k < SRC32_PER_THREAD; k++) {
-      const int q = tid + k * SRC_TPB;
+      const int q = tid + k * SRC32_TPB;
       ls[k] = lvr[k] = lvt[k] = 0.0;
       if (q < hr * HC) {
         const int r = q / HC, cc = q - r * HC, i = GI(r);
@@ -1249,7 +1247,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     // reference vtheta) of rows ln and ln + 64
     {
       const int w = tid >> 6, ln = tid & 63;
-      for (int r = ln; r < hr; r += 64) {
+      for (int r = ln; w < 4 && r < hr; r += 64) {
         const int gi = GI(r), i = min(max(gi, 0), nr - 1);
         const double v0 = ring_table(t, a.nu, w)[i], v1 = ring_table(t, a.nu, w + 4)[i];
         const double v2 = ring_table(t, a.nu, w + 8)[i];
@@ -1261,7 +1259,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
         if (w == 0) rt[RT_EVZ][r] = (double)ez;
         if (w == 1) rt[RT_VREF][r] = vref;
       }
-      for (int c = tid; c < HC; c += SRC_TPB) {
+      for (int c = tid; c < HC; c += SRC32_TPB) {
         const int j = GJ(c);
         ct[0][c] = t.cos_pot[j];
         ct[1][c] = t.sin_pot[j];
@@ -1269,7 +1267,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     }
     // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004)
     if (a.evanescent) {
-      for (int r = tid; r < hr; r += SRC_TPB) {
+      for (int r = tid; r < hr; r += SRC32_TPB) {
         const int i = GI(r);
         if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
       }
@@ -1277,7 +1275,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     __syncthreads();
 #pragma unroll
   for (int k = 0; k < SRC32_PER_THREAD; k++) {
-    const int q = tid + k * SRC_TPB;
+    const int q = tid + k * SRC32_TPB;
     if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
@@ -1309,7 +1307,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   {
 #pragma unroll
     for (int k = 0; k < SRC32_PER_THREAD; k++) {
-      const int q = tid + k * SRC_TPB;
+      const int q = tid + k * SRC32_TPB;
       if (q >= hr * HC) break;
       int r = q / HC, cc = q - r * HC;
       int i = GI(r);
@@ -1344,7 +1342,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < SRC32_PER_THREAD; k++) {
-      const int q = tid + k * SRC_TPB;
+      const int q = tid + k * SRC32_TPB;
       if (q >= hr * HC) break;
       int r = q / HC, cc = q - r * HC;
       sA[r][cc] = regA[k];
@@ -1358,7 +1356,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   const float onethird = 1.0f / 3.0f;
 #pragma unroll
   for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * SRC32_TPB;
     if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
@@ -1387,7 +1385,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   // ---- S3: viscous velocity update (Kernels.cu:118-150) ----
 #pragma unroll
   for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * SRC32_TPB;
     if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
@@ -1415,7 +1413,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   // ---- S4: artificial pressure (Kernels.cu:409-434) ----
 #pragma unroll
   for (int kq = 0; kq < SRC32_PER_THREAD; kq++) {
-    const int q = tid + kq * SRC_TPB;
+    const int q = tid + kq * SRC32_TPB;
     if (q >= hr * HC) break;
     int r = q / HC, cc = q - r * HC;
     int i = GI(r);
@@ -1428,10 +1426,10 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
   double os[SRC32_OWN_MAX], ovr[SRC32_OWN_MAX], ovt[SRC32_OWN_MAX];
 #pragma unroll
-  for (int k = 0; k < SRC32_OWN_MAX; k++) own_load(a, i0, j0, th, tid + k * SRC_TPB, os[k], ovr[k], ovt[k]);
+  for (int k = 0; k < SRC32_OWN_MAX; k++) own_load(a, i0, j0, th, tid + k * SRC32_TPB, os[k], ovr[k], ovt[k]);
 #pragma unroll
   for (int k = 0; k < SRC32_OWN_MAX; k++) {
-    const int q = tid + k * SRC_TPB;
+    const int q = tid + k * SRC32_TPB;
     if (q >= th * TW) break;
     int ro = q / TW, co = q % TW;
     int r = ro + HALO_R, cc = co + HALO_C;
@@ -1468,7 +1466,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
 #undef GJ
 }
 
-__global__ __launch_bounds__(SRC_TPB) void k_src32(src32_args a, int th) {
+__global__ __launch_bounds__(SRC32_TPB) void k_src32(src32_args a, int th) {
   extern __shared__ float lds_src32[];
   src32_tile(a, blockIdx.x, blockIdx.y, th, lds_src32);
 }
@@ -1812,7 +1810,7 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     const int th = c->tile_src_th;
     FG_HIP(hipFuncSetAttribute((const void *)k_src32, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)src32_lds_bytes(SRC32_MAX_TH)));
-    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC_TPB),
+    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC32_TPB),
                        src32_lds_bytes(th), c->stream, s3, th);
   } else {
     hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC64_TPB), 0, c->stream,
@@ -1960,7 +1958,7 @@ __device__ __forceinline__ int2 uniform2(int2 w) {
 // one work item per workgroup: without a loop's carried scalars the tile's
 // uniform arguments fit the SGPRs instead of spilling into VGPR lanes
 // (253 -> 112 VGPRs)
-__global__ __launch_bounds__(SRC_TPB) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
+__global__ __launch_bounds__(SRC32_TPB) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
   extern __shared__ float lds_bsrc[];
   if ((int)blockIdx.x >= n) return;
   const int4 w = uniform4(items[blockIdx.x]);
@@ -2148,7 +2146,7 @@ static fg_status batch_build(fg_batch_dev *b) {
   FG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   auto grid = [](int n) { return dim3(n < BATCH_CAP_BLOCKS ? (n > 0 ? n : 1) : BATCH_CAP_BLOCKS); };
   (void)hipMemcpyAsync(b->d_disk, b->h_disk, b->n * sizeof(batch_disk), hipMemcpyHostToDevice, st);
-  hipLaunchKernelGGL(k_batch_src, dim3(b->n_src > 0 ? b->n_src : 1), dim3(SRC_TPB), src32_lds_bytes(b->max_th), st,
+  hipLaunchKernelGGL(k_batch_src, dim3(b->n_src > 0 ? b->n_src : 1), dim3(SRC32_TPB), src32_lds_bytes(b->max_th), st,
                      b->d_disk, b->it_src, b->n_src);
   hipLaunchKernelGGL(k_batch_trad, grid(b->n_trad), dim3(TRAD_TPB), 0, st, b->d_disk, b->it_trad, b->n_trad);
   hipLaunchKernelGGL(k_batch_taz, grid(b->n_taz), dim3(64), 0, st, b->d_disk, b->it_taz, b->n_taz);
