@@ -945,6 +945,7 @@ __global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr
 // one ring; ring holds NSEC doubles (the ring's final vtheta).  FIN_B
 // cells per thread have their loads issued together (4 for the single-disk
 // kernel; 1 in the batched one, which would otherwise lose a wave per SIMD)
+#define FIN_OV_K 16  // cells per thread of the overlapped CFL pass (rings up to 3072 cells)
 template <int FIN_B>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[256];
@@ -1037,6 +1038,90 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     }
   }
   __syncthreads();
+  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
+  const double dxrad = t.Rsup[i] - t.Rinf[i];
+  const double dxtheta = t.dxtheta_cfl[i];
+  const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
+  const double cs = t.cs_iso[i];
+  const bool radial_n = i + 1 < nr;  // vrad of ring i+1 (0 on the ghost ring)
+  double best = 1e30;
+  // Rings of up to FIN_OV_K * 192 cells: lane 0 of wave 0 forms the ring
+  // mean while waves 1-3 load and evaluate the parts of each cell's time
+  // step that do not depend on it, the radial, sound-speed and shear terms
+  // (A = invdt1^2 + invdt2^2, B = invdt4^2 below), so the latency of the
+  // sequential sum overlaps the CFL pass's loads.  The sum of squares then
+  // completes as (A + invdt3^2) + B, the reference's left-to-right order.
+  if (FIN_B == 4 && ns <= 192 * FIN_OV_K) {
+    double A[FIN_OV_K], B[FIN_OV_K];
+    if (tid == 0) {
+      double m = seq_sum_lds(ring, ns);
+      m /= (double)ns;
+      a.vmoy[i] = m;
+      s_vmoy = m;
+    } else if (tid >= 64 && i > 0) {
+      const double invdt1 = cs / mindx;
+#pragma unroll
+      for (int u0 = 0; u0 < FIN_OV_K; u0 += 4) {
+        double pvr[4], prp[4], prm[4], pd0[4], pd1[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = tid - 64 + (u0 + u) * 192;
+          pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = 0.0;
+          if (j < ns) {
+            const long l = base + j;
+            pvr[u] = a.vr[l];
+            if (radial_n) {
+              prp[u] = a.xrmp[l];
+              prm[u] = a.xrmm[l + ns];
+              pd1[u] = a.xd[l + ns];
+              pd0[u] = a.xd[l];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = tid - 64 + (u0 + u) * 192;
+          A[u0 + u] = B[u0 + u] = 0.0;
+          if (j >= ns) continue;
+          const double vr = pvr[u];
+          double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
+          if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+          const double invdt2 = fabs(vr) / dxrad;
+          double dvr = vrn - vr;
+          double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
+          dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
+          dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
+          double x = dvr / dxrad, y = dvt / dxtheta;
+          double invdt4 = (y > x) ? y : x;
+          invdt4 *= 4.0 * 1.41 * 1.41;
+          A[u0 + u] = invdt1 * invdt1 + invdt2 * invdt2;
+          B[u0 + u] = invdt4 * invdt4;
+        }
+      }
+    }
+    __syncthreads();
+    if (i == 0) return;
+    if (tid >= 64) {
+      const double vm = s_vmoy;
+#pragma unroll
+      for (int u = 0; u < FIN_OV_K; u++) {
+        const int j = tid - 64 + u * 192;
+        if (j >= ns) break;
+        const double vres = a.fast ? (ring[j] - vm) : ring[j];
+        const double invdt3 = fabs(vres) / dxtheta;
+        const double dtc = 0.5 / sqrt((A[u] + invdt3 * invdt3) + B[u]);
+        if (dtc < best) best = dtc;
+      }
+    }
+    red[tid] = best;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.newdt[i] = red[0];
+    return;
+  }
   if (tid == 0) {
     double m = seq_sum_lds(ring, ns);
     m /= (double)ns;
@@ -1045,13 +1130,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   }
   __syncthreads();
   if (i == 0) return;
-  // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
-  const double dxrad = t.Rsup[i] - t.Rinf[i];
-  const double dxtheta = t.dxtheta_cfl[i];
-  const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
-  const double vm = s_vmoy, cs = t.cs_iso[i];
-  double best = 1e30;
-  const bool radial_n = i + 1 < nr;  // vrad of ring i+1 (0 on the ghost ring)
+  const double vm = s_vmoy;
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B];
 #pragma unroll
