@@ -389,24 +389,29 @@ struct trad_args {
 };
 
 // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081) for one ring
-// by one wave: lanes stream the ring through LDS 64 sectors at a time
-// (next chunk prefetched into registers) and lane 0 adds them in j order.
+// by one wave: the lanes stage the ring through LDS 512 sectors at a time
+// (the next 512 loaded into registers meanwhile) and lane 0 adds them in j
+// order with its LDS reads issued ahead of the dependent adds.  One wave:
+// the barrier only orders the LDS accesses, and lane 0 has finished a
+// round's sum before the next round's stores in program order.
+#define RMW_CH 8  // sectors per lane per round
 __device__ void ring_mean_wave(const double *v, int ns, double *out) {
-  __shared__ double buf[2][64];
+  __shared__ double buf[64 * RMW_CH];
   const int lane = threadIdx.x;
-  const int nch = (ns + 63) / 64;
-  double next = (lane < ns) ? v[lane] : 0.0;
+  const int nch = (ns + 64 * RMW_CH - 1) / (64 * RMW_CH);
+  double nx[RMW_CH];
+#pragma unroll
+  for (int u = 0; u < RMW_CH; u++) nx[u] = (lane + 64 * u < ns) ? v[lane + 64 * u] : 0.0;
   double sum = 0.0;
   for (int c = 0; c < nch; c++) {
-    double cur = next;
-    int e = (c + 1) * 64 + lane;
-    next = (c + 1 < nch && e < ns) ? v[e] : 0.0;
-    buf[c & 1][lane] = cur;
+#pragma unroll
+    for (int u = 0; u < RMW_CH; u++) buf[lane + 64 * u] = nx[u];
+    const int e0 = (c + 1) * 64 * RMW_CH + lane;
+#pragma unroll
+    for (int u = 0; u < RMW_CH; u++) nx[u] = (e0 + 64 * u < ns) ? v[e0 + 64 * u] : 0.0;
     __syncthreads();
-    if (lane == 0) {
-      int n = min(64, ns - c * 64);
-      for (int k = 0; k < n; k++) sum += buf[c & 1][k];
-    }
+    if (lane == 0) sum = seq_sum_lds(buf, min(64 * RMW_CH, ns - c * 64 * RMW_CH), sum);
+    __syncthreads();
   }
   if (lane == 0) *out = sum / (double)ns;
 }
@@ -1051,7 +1056,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   // (A = invdt1^2 + invdt2^2, B = invdt4^2 below), so the latency of the
   // sequential sum overlaps the CFL pass's loads.  The sum of squares then
   // completes as (A + invdt3^2) + B, the reference's left-to-right order.
-  if (FIN_B == 4 && ns <= 192 * FIN_OV_K) {
+  if (FIN_B > 1 && ns <= 192 * FIN_OV_K) {
     double A[FIN_OV_K], B[FIN_OV_K];
     if (tid == 0) {
       double m = seq_sum_lds(ring, ns);

@@ -201,12 +201,12 @@ __device__ __forceinline__ void dreduce_block4(const Src &src, int k, long n, in
   }
 }
 
-// Left-to-right sum of ring[0..n) in one lane, the order of the
-// reference's one-thread ring loops (Kernels.cu:1276-1279).  Values come
-// from LDS in blocks of 16, the next block's loads issued before the
-// current block's adds, so the LDS latency overlaps the dependent adds.
-__device__ __forceinline__ double seq_sum_lds(const double *ring, int n) {
-  double m = 0.0;
+// Left-to-right sum m + ring[0] + ring[1] + ... + ring[n-1] in one lane,
+// the order of the reference's one-thread ring loops (Kernels.cu:1276-1279).
+// Values come from LDS in blocks of 16, the next block's loads issued
+// before the current block's adds, so the LDS latency overlaps the
+// dependent adds.
+__device__ __forceinline__ double seq_sum_lds(const double *ring, int n, double m = 0.0) {
   int j = 0;
   if (n >= 32) {
     double a[16], b[16];
