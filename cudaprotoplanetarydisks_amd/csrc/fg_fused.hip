@@ -375,7 +375,6 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
 // face flux F(k) = dt*dtheta*Rinf[k]*Q*(k)*Sigma*(k)*vrad(k) serves both
 // neighbouring rings (Rsup[k-1] == Rinf[k]).
 #define TRAD_TPB 64
-#define TRAD_BAND 16
 #define NQ 5  // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Sigma
 
 struct trad_args {
@@ -385,7 +384,7 @@ struct trad_args {
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, nstrips;
-  int band;  // rings per column strip (FP32 sweep; the FP64 sweep uses TRAD_BAND)
+  int band;  // rings per column strip
 };
 
 // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081) for one ring
@@ -464,8 +463,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   // that column
   const int j = min((bid % a.nstrips) * TRAD_TPB + (int)threadIdx.x, ns - 1);
   const int jp = (j + 1) % ns;
-  const int b0 = (bid / a.nstrips) * TRAD_BAND;
-  const int b1 = min(b0 + TRAD_BAND, nr);
+  const int b0 = (bid / a.nstrips) * a.band;
+  const int b1 = min(b0 + a.band, nr);
   const fg_dev_tables &t = a.t;
   const double dt = a.dt, OF = a.omega_frame;
   const double cflux = dt * t.dtheta;
@@ -1918,7 +1917,7 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.ns = ns;
   ta.vmed = c->VMed;
   ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
-  ta.band = fp32 ? c->tile_trad_band : TRAD_BAND;
+  ta.band = c->tile_trad_band;
   prof_mark(c);
   const dim3 gtrad(nr + ta.nstrips * ((nr + ta.band - 1) / ta.band));
   if (fp32)
