@@ -10,6 +10,31 @@
 
 #define FG_DEV __device__ __forceinline__
 
+// n / d for many n with one d: the compiler's FP64 division (v_div_scale,
+// v_rcp_f64 and two Newton steps on the scaled denominator, the corrected
+// quotient, v_div_fmas, v_div_fixup) with the denominator's half done once.
+// Neither operand is scaled and the last fma stands for v_div_fmas, so the
+// result is bit-identical to n / d whenever v_div_scale would scale
+// neither: d and 1/d normal, and n zero, infinite, NaN or finite with
+// 2^-969 <= |n| and |n / d| < 2^767 (v_div_fixup returns the same signed
+// zero, infinity or NaN as the full sequence).
+struct recip64 {
+  double d, r;
+  FG_DEV explicit recip64(double den) : d(den) {
+    r = __builtin_amdgcn_rcp(den);
+    double t = fma(-den, r, 1.0);
+    r = fma(r, t, r);
+    t = fma(-den, r, 1.0);
+    r = fma(r, t, r);
+  }
+  FG_DEV recip64(double den, double refined) : d(den), r(refined) {}  // r of an earlier recip64(den)
+  FG_DEV double div(double n) const {
+    const double q0 = n * r;
+    const double e = fma(-d, q0, n);
+    return __builtin_amdgcn_div_fixup(fma(e, r, q0), d, n);
+  }
+};
+
 // van Leer radial slope, Kernels.cu:808-818
 FG_DEV double vl_rad_slope(double qm, double q0, double qp, double idrm_i, double idrm_ip) {
   double dqm = (q0 - qm) * idrm_i;

@@ -522,7 +522,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
       Q[4][2] = s;
       sg[2] = s;
       vr[2] = v0;
-      for (int q = 0; q < NQ; q++) W[q][2] = Q[q][2] / (s + 1e-20);
+      const recip64 rs(s + 1e-20);  // Work = Q / (Sigma + 1e-20), one denominator
+      for (int q = 0; q < NQ; q++) W[q][2] = rs.div(Q[q][2]);
     } else {
       for (int q = 0; q < NQ; q++) Q[q][2] = W[q][2] = 0.0;
       sg[2] = 0.0;
@@ -658,7 +659,11 @@ __device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&V
     for (int c = 0; c < CL; c++)
       DS[c] = vl_theta_star(V(c), dt, c ? dens[c - 1] : Wm, dens[c], c ? D[c - 1] : Dm, D[c], dxtheta);
   }
-  // VanLeerTheta of each quantity, Sigma last (TransportEuler.cu:234-243)
+  // VanLeerTheta of each quantity, Sigma last (TransportEuler.cu:234-243);
+  // the Work divisions of a cell share its denominator Sigma + 1e-20
+  double rd[CL];
+#pragma unroll
+  for (int c = 0; c < CL; c++) rd[c] = recip64(dens[c] + 1e-20).r;
 #pragma unroll
   for (int qq = 0; qq < NQ; qq++) {
     // one quantity at a time: keeps the live registers to this quantity's
@@ -666,7 +671,7 @@ __device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&V
     __builtin_amdgcn_sched_barrier(0);
     double W[CL], D[CL], F[CL];
 #pragma unroll
-    for (int c = 0; c < CL; c++) W[c] = q[qq][c] / (dens[c] + 1e-20);
+    for (int c = 0; c < CL; c++) W[c] = recip64(dens[c] + 1e-20, rd[c]).div(q[qq][c]);
     const double Wm = lane_prev(W[CL - 1]), Wp = lane_next(W[0]);
 #pragma unroll
     for (int c = 0; c < CL; c++)
@@ -809,13 +814,14 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   // vtheta from the angular momenta (Kernels.cu:1206-1207): cells j - 1
   // and j of the unshifted ring
   const double Rm = t.Rmed[i];
+  const recip64 rRm(Rm);
   double vt[CL];
   {
     const double Wm = lane_prev(q[2][CL - 1]), Sm = lane_prev(q[NQ - 1][CL - 1]);
 #pragma unroll
     for (int c = 0; c < CL; c++) {
       const double s = q[NQ - 1][c];
-      vt[c] = ((c ? q[2][c - 1] : Wm) + q[3][c]) / (s + (c ? q[NQ - 1][c - 1] : Sm) + 1e-15) / Rm -
+      vt[c] = rRm.div(((c ? q[2][c - 1] : Wm) + q[3][c]) / (s + (c ? q[NQ - 1][c - 1] : Sm) + 1e-15)) -
               Rm * a.omega_frame;
     }
   }
@@ -1064,6 +1070,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       s_vmoy = m;
     } else if (tid >= 64 && i > 0) {
       const double invdt1 = cs / mindx;
+      const recip64 rdr(dxrad), rdt(dxtheta);
 #pragma unroll
       for (int u0 = 0; u0 < FIN_OV_K; u0 += 4) {
         double pvr[4], prp[4], prm[4], pd0[4], pd1[4];
@@ -1090,12 +1097,12 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
           const double vr = pvr[u];
           double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
           if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
-          const double invdt2 = fabs(vr) / dxrad;
+          const double invdt2 = rdr.div(fabs(vr));
           double dvr = vrn - vr;
           double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
           dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
           dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
-          double x = dvr / dxrad, y = dvt / dxtheta;
+          double x = rdr.div(dvr), y = rdt.div(dvt);
           double invdt4 = (y > x) ? y : x;
           invdt4 *= 4.0 * 1.41 * 1.41;
           A[u0 + u] = invdt1 * invdt1 + invdt2 * invdt2;
@@ -1107,12 +1114,13 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     if (i == 0) return;
     if (tid >= 64) {
       const double vm = s_vmoy;
+      const recip64 rdt(dxtheta);
 #pragma unroll
       for (int u = 0; u < FIN_OV_K; u++) {
         const int j = tid - 64 + u * 192;
         if (j >= ns) break;
         const double vres = a.fast ? (ring[j] - vm) : ring[j];
-        const double invdt3 = fabs(vres) / dxtheta;
+        const double invdt3 = rdt.div(fabs(vres));
         const double dtc = 0.5 / sqrt((A[u] + invdt3 * invdt3) + B[u]);
         if (dtc < best) best = dtc;
       }
@@ -1135,6 +1143,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   __syncthreads();
   if (i == 0) return;
   const double vm = s_vmoy;
+  const recip64 rdr(dxrad), rdt(dxtheta);
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B];
 #pragma unroll
@@ -1161,13 +1170,13 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
     double vres = a.fast ? (ring[j] - vm) : ring[j];
     double invdt1 = cs / mindx;
-    double invdt2 = fabs(vr) / dxrad;
-    double invdt3 = fabs(vres) / dxtheta;
+    double invdt2 = rdr.div(fabs(vr));
+    double invdt3 = rdt.div(fabs(vres));
     double dvr = vrn - vr;
     double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
     dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
     dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
-    double x = dvr / dxrad, y = dvt / dxtheta;
+    double x = rdr.div(dvr), y = rdt.div(dvt);
     double invdt4 = (y > x) ? y : x;
     invdt4 *= 4.0 * 1.41 * 1.41;
     double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
