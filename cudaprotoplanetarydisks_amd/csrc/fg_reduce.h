@@ -23,9 +23,10 @@
 
 // ring and sector of flattened cell e: a 32-bit division (one float
 // reciprocal and corrections) below 2^31 cells, the emulated 64-bit one
-// beyond
+// beyond; NARROW: the caller knows e < 2^31 (no branch)
+template <bool NARROW = false>
 __device__ __forceinline__ void cell_ij(long e, int ns, int &i, int &j) {
-  if (e < 0x80000000L) {
+  if (NARROW || e < 0x80000000L) {
     const unsigned ue = (unsigned)e, q = ue / (unsigned)ns;
     i = (int)q;
     j = (int)(ue - q * (unsigned)ns);
@@ -48,9 +49,10 @@ struct force_src {
   const double *dens, *Surf, *Rmed, *cosc, *sinc;
   double x, y, rs;
   int ns;
+  template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
     int i, j;
-    cell_ij(e, ns, i, j);
+    cell_ij<NARROW>(e, ns, i, j);
     double cellmass = Surf[i] * dens[e];
     double dx = Rmed[i] * cosc[j] - x;
     double dy = Rmed[i] * sinc[j] - y;
@@ -102,9 +104,10 @@ struct cpm_src {
   const double *dens, *Surf, *Rmed, *cosc, *sinc;
   double xpl, ypl, hill;
   int ns;
+  template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &m, double &zero) const {
     int i, j;
-    cell_ij(e, ns, i, j);
+    cell_ij<NARROW>(e, ns, i, j);
     double dx = Rmed[i] * cosc[j] - xpl;
     double dy = Rmed[i] * sinc[j] - ypl;
     double dist = sqrt(dx * dx + dy * dy);
@@ -115,6 +118,7 @@ struct cpm_src {
 
 struct array_src {
   const double *a, *b;  // b may be NULL
+  template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
     fx = a[e];
     fy = b ? b[e] : 0.0;
@@ -123,35 +127,59 @@ struct array_src {
 
 // block blk of B (a RED_TPB-thread workgroup); partial[blk] and
 // partial[B + blk] receive the x and y block sums
-// CH summand rows are staged per round (CH * 64 / RED_TPB per thread, all
-// evaluated before the round's barrier)
+// CH summand rows are staged per round (CH * 64 / RED_TPB per thread).  A
+// round's summands are evaluated branch-free (indices past n evaluate
+// cell n - 1 and are dropped), so all their loads issue together; wave 0
+// then reads its LDS values 8 rows ahead of the dependent adds.
+template <bool NARROW, class Src, int CH>
+__device__ __forceinline__ void dreduce_stage(const Src &src, long n, long b64, long G, long k0, double (*vx)[64],
+                                              double (*vy)[64]) {
+#pragma unroll
+  for (int c0 = 0; c0 < CH * 64 / RED_TPB; c0++) {
+    const int c = threadIdx.x + c0 * RED_TPB;
+    const int kk = c >> 6, lane = c & 63;
+    const long e = b64 + (k0 + kk) * G + lane;
+    const bool ok = e < n;
+    double fx, fy;
+    src.template operator()<NARROW>(ok ? e : n - 1, fx, fy);
+    vx[kk][lane] = ok ? fx : 0.0;
+    vy[kk][lane] = ok ? fy : 0.0;
+  }
+}
+
 template <class Src, int CH = RED_CH>
 __device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int blk, double *partial) {
+  static_assert(CH % 8 == 0, "rows are added in groups of 8");
   __shared__ double vx[CH][64];
   __shared__ double vy[CH][64];
   const long G = 64L * B;
   const long b64 = 64L * blk;
+  const bool narrow = n <= 0x80000000L;  // every cell index below 2^31
   double acc = 0.0;
   for (long k0 = 0; b64 + k0 * G < n; k0 += CH) {
-#pragma unroll
-    for (int c0 = 0; c0 < CH * 64 / RED_TPB; c0++) {
-      const int c = threadIdx.x + c0 * RED_TPB;
-      int kk = c >> 6, lane = c & 63;
-      long e = b64 + (k0 + kk) * G + lane;
-      double fx = 0.0, fy = 0.0;
-      if (e < n) src(e, fx, fy);
-      vx[kk][lane] = fx;
-      vy[kk][lane] = fy;
-    }
+    if (narrow)
+      dreduce_stage<true, Src, CH>(src, n, b64, G, k0, vx, vy);
+    else
+      dreduce_stage<false, Src, CH>(src, n, b64, G, k0, vx, vy);
     __syncthreads();
     if (threadIdx.x < 64) {
       const int t = threadIdx.x & 31;
       const bool isy = threadIdx.x >= 32;
-      for (int kk = 0; kk < CH; kk++) {
-        long e = b64 + (k0 + kk) * G + t;
-        if (e < n) {
-          acc += isy ? vy[kk][t] : vx[kk][t];
-          if (e + 32 < n) acc += isy ? vy[kk][t + 32] : vx[kk][t + 32];
+      const double(*v)[64] = isy ? vy : vx;
+      for (int g = 0; g < CH; g += 8) {
+        double lo[8], hi[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          lo[kk] = v[g + kk][t];
+          hi[kk] = v[g + kk][t + 32];
+        }
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          const long e = b64 + (k0 + g + kk) * G + t;
+          if (e < n) {
+            acc += lo[kk];
+            if (e + 32 < n) acc += hi[kk];
+          }
         }
       }
     }
