@@ -468,9 +468,12 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   const fg_dev_tables &t = a.t;
   const double dt = a.dt, OF = a.omega_frame;
   const double cflux = dt * t.dtheta;
-  // rolling window: index 0 = ring m-2, 1 = m-1, 2 = m
-  double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][2] = {}, F[NQ][2] = {};
-  double sg[3] = {}, dqs[2] = {}, vr[3] = {};
+  // windows over the last three rings: ring x (its Q, W, Sigma, vrad and
+  // slope) and face x (its flux) live in slot x % 3, so a step writes its
+  // new ring over the one three back and nothing is shifted; the step for
+  // ring m is instantiated per m % 3 to keep the slots static
+  double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][3] = {}, F[NQ][3] = {};
+  double sg[3] = {}, dqs[3] = {}, vr[3] = {};
   const int m0 = max(b0 - 2, 0);
   ring_lanes rt;
   rt.load(t, m0 - 2, nr);
@@ -488,7 +491,11 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     n_vt = a.vt[l];
     asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
   }
-  auto ring = [&](int m, auto store) {
+  // inner: the caller guarantees 2 <= m < nr (ring m inside the disk, slope
+  // and face of ring m - 1 interior), which drops the edge cases
+  auto ring = [&](int m, auto store, auto phase, auto inner) {
+    constexpr int C = decltype(phase)::value, B1 = (C + 2) % 3, B2 = (C + 1) % 3;  // rings m, m-1, m-2
+    constexpr bool IN = decltype(inner)::value;
     const int r = rt.at(t, m, nr);
     const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
     {
@@ -499,83 +506,103 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
       n_vtp = a.vt[l + jp];
       n_vt = a.vt[l + j];
     }
-    // shift the window
-    for (int q = 0; q < NQ; q++) {
-      Q[q][0] = Q[q][1];
-      Q[q][1] = Q[q][2];
-      W[q][0] = W[q][1];
-      W[q][1] = W[q][2];
-    }
-    sg[0] = sg[1];
-    sg[1] = sg[2];
-    vr[0] = vr[1];
-    vr[1] = vr[2];
     // ring tables: Rmed[m], Rmed[m-1], Rmed[m-2] at lanes r, r-1, r-2
     const double Rm = lane_get(rt.Rmed, r);
-    if (m < nr) {
+    if (IN || m < nr) {
       const double s = c_s;
       const double v0 = c_v0, v1 = c_v1;
-      Q[0][2] = s * v1;
-      Q[1][2] = s * v0;
-      Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
-      Q[3][2] = s * (c_vt + Rm * OF) * Rm;
-      Q[4][2] = s;
-      sg[2] = s;
-      vr[2] = v0;
+      Q[0][C] = s * v1;
+      Q[1][C] = s * v0;
+      Q[2][C] = s * (c_vtp + Rm * OF) * Rm;
+      Q[3][C] = s * (c_vt + Rm * OF) * Rm;
+      Q[4][C] = s;
+      sg[C] = s;
+      vr[C] = v0;
       const recip64 rs(s + 1e-20);  // Work = Q / (Sigma + 1e-20), one denominator
-      for (int q = 0; q < NQ; q++) W[q][2] = rs.div(Q[q][2]);
+      for (int q = 0; q < NQ; q++) W[q][C] = rs.div(Q[q][C]);
     } else {
-      for (int q = 0; q < NQ; q++) Q[q][2] = W[q][2] = 0.0;
-      sg[2] = 0.0;
-      vr[2] = 0.0;  // vrad of the ghost ring
+      for (int q = 0; q < NQ; q++) Q[q][C] = W[q][C] = 0.0;
+      sg[C] = 0.0;
+      vr[C] = 0.0;  // vrad of the ghost ring
     }
-    // slope of ring m-1 (zero on rings 0 and NRAD-1)
+    // slope of ring k = m-1 (zero on rings 0 and NRAD-1)
     const int k = m - 1;
     if (!decltype(store)::value && k < 0) return;  // storing steps have k >= 1
-    const bool flat = (k == 0 || k >= nr - 1);
+    const bool flat = !IN && (k == 0 || k >= nr - 1);
     const double idm0 = lane_get(rt.idm, r - 1), idm1 = lane_get(rt.idm, r);
-    for (int q = 0; q < NQ; q++) {
-      dq[q][0] = dq[q][1];
-      dq[q][1] = flat ? 0.0 : vl_rad_slope(W[q][0], W[q][1], W[q][2], idm0, idm1);
-    }
-    dqs[0] = dqs[1];
-    dqs[1] = flat ? 0.0 : vl_rad_slope(sg[0], sg[1], sg[2], idm0, idm1);
+    for (int q = 0; q < NQ; q++) dq[q][B1] = flat ? 0.0 : vl_rad_slope(W[q][B2], W[q][B1], W[q][C], idm0, idm1);
+    dqs[B1] = flat ? 0.0 : vl_rad_slope(sg[B2], sg[B1], sg[C], idm0, idm1);
     // face k: upwind values (StarRadKernel2); faces 0 and NRAD are zero
     double DS = 0.0;
     double QS[NQ];
-    if (k > 0 && k < nr) {
-      const double vk = vr[1];
+    if (IN || (k > 0 && k < nr)) {
+      const double vk = vr[B1];
       const double Rkm = lane_get(rt.Rmed, r - 2), Rk = lane_get(rt.Rmed, r - 1);
-      DS = vl_rad_star(vk, dt, sg[0], sg[1], dqs[0], dqs[1], Rkm, Rk, Rm, k);
-      for (int q = 0; q < NQ; q++) QS[q] = vl_rad_star(vk, dt, W[q][0], W[q][1], dq[q][0], dq[q][1], Rkm, Rk, Rm, k);
+      DS = vl_rad_star(vk, dt, sg[B2], sg[B1], dqs[B2], dqs[B1], Rkm, Rk, Rm, k);
+      for (int q = 0; q < NQ; q++)
+        QS[q] = vl_rad_star(vk, dt, W[q][B2], W[q][B1], dq[q][B2], dq[q][B1], Rkm, Rk, Rm, k);
     } else {
       for (int q = 0; q < NQ; q++) QS[q] = 0.0;
     }
-    const double vface = (k < nr) ? vr[1] : 0.0;
+    const double vface = (IN || k < nr) ? vr[B1] : 0.0;
     const double Rinf = lane_get(rt.Rinf, r - 1);
-    for (int q = 0; q < NQ; q++) {
-      F[q][0] = F[q][1];
-      F[q][1] = cflux * Rinf * QS[q] * DS * vface;
-    }
+    for (int q = 0; q < NQ; q++) F[q][B1] = cflux * Rinf * QS[q] * DS * vface;
     // ring k-1 is complete: Q += (F(k-1) - F(k)) / Surf
     if (decltype(store)::value) {
       const int o = k - 1;
       const long lo = (long)o * ns + j;
       const double isurf = lane_get(rt.isurf, r - 2);
       for (int q = 0; q < NQ; q++) {
-        double varq = F[q][0];
-        varq -= F[q][1];
-        a.out[q][lo] = Q[q][0] + varq * isurf;
+        double varq = F[q][B2];
+        varq -= F[q][B1];
+        a.out[q][lo] = Q[q][B2] + varq * isurf;
       }
     }
   };
+  // a single step in the rotating layout (newest ring in slot 2): shift
+  // every window by one, then the step of phase 2.  Three consecutive
+  // steps of phases 0, 1, 2 leave the windows in that same layout.
+  auto shift = [&]() {
+    for (int q = 0; q < NQ; q++) {
+      Q[q][0] = Q[q][1];
+      Q[q][1] = Q[q][2];
+      W[q][0] = W[q][1];
+      W[q][1] = W[q][2];
+      dq[q][0] = dq[q][1];
+      F[q][0] = F[q][1];
+    }
+    sg[0] = sg[1];
+    sg[1] = sg[2];
+    vr[0] = vr[1];
+    vr[1] = vr[2];
+    dqs[0] = dqs[1];
+  };
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
+  using P2 = std::integral_constant<int, 2>;
   // rings m0 .. b0 + 1 fill the window; from m = b0 + 2 on ring m - 2 of
   // the band is complete after every step
+  using T = std::true_type;
+  using F0 = std::false_type;
   int m = m0;
-  for (; m < b0 + 2; m++) ring(m, std::false_type());
-  // the storing loop starts with nothing in flight from the filling one
+  for (; m < b0 + 2; m++) {
+    shift();
+    ring(m, F0(), P2(), F0());
+  }
+  // the storing loop starts with nothing in flight from the filling one;
+  // its steps have m >= b0 + 2 >= 2, so those below nr are inner
   asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
-  for (; m <= b1 + 1; m++) ring(m, std::true_type());
+  const int mi = min(b1 + 1, nr - 1);
+  for (; m + 2 <= mi; m += 3) {
+    ring(m, T(), P0(), T());
+    ring(m + 1, T(), P1(), T());
+    ring(m + 2, T(), P2(), T());
+  }
+  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+  for (; m <= b1 + 1; m++) {
+    shift();
+    ring(m, T(), P2(), F0());
+  }
 }
 
 // ------------------------------------------------------------ k_taz ----
