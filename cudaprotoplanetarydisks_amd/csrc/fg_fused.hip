@@ -2444,3 +2444,41 @@ extern "C" fg_status fg_set_tiles(fg_ctx *c, int src_rows, int trad_band) {
   c->tile_trad_band = trad_band;
   return FG_OK;
 }
+
+// ------------------------------------------------- shared-divisor check ----
+// bitwise comparison of recip64(d).div(n) with the division n / d
+__global__ void k_check_recip64(const double *num, const double *den, long n, unsigned long long *bad) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double q = num[t] / den[t];
+  const double s = recip64(den[t]).div(num[t]);
+  if (__double_as_longlong(q) != __double_as_longlong(s)) atomicAdd(bad, 1ull);
+}
+
+extern "C" fg_status fg_check_shared_division(const double *num, const double *den, long n, long *mismatches) {
+  if (!num || !den || !mismatches || n < 0) return FG_E_INVALID;
+  *mismatches = 0;
+  if (n == 0) return FG_OK;
+  double *dn = nullptr, *dd = nullptr;
+  unsigned long long *dbad = nullptr, hbad = 0;
+  hipError_t e = hipMalloc((void **)&dn, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&dd, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&dbad, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemcpy(dn, num, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dd, den, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dbad, &hbad, sizeof(hbad), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_check_recip64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dn, dd, n, dbad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&hbad, dbad, sizeof(hbad), hipMemcpyDeviceToHost);
+  if (dn) (void)hipFree(dn);
+  if (dd) (void)hipFree(dd);
+  if (dbad) (void)hipFree(dbad);
+  if (e != hipSuccess) {
+    fg_set_error("fg_check_shared_division: %s", hipGetErrorString(e));
+    return FG_E_HIP;
+  }
+  *mismatches = (long)hbad;
+  return FG_OK;
+}

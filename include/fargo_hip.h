@@ -224,6 +224,14 @@ fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_
  * No reference counterpart (the reference's launch shape is fixed,
  * Main.cu:204-226). */
 fg_status fg_set_tiles(fg_ctx *ctx, int src_rows, int trad_band);
+/* No counterpart in the reference: a self-check of the FP64 division with a
+ * shared denominator that the fused kernels use (one reciprocal refinement
+ * of d for several numerators).  Computes num[k] / den[k] both ways on the
+ * GPU for k < n (host arrays) and stores the number of results whose bits
+ * differ in *mismatches.  The two agree for d and 1/d normal and numerators
+ * that are zero, infinite, NaN or finite with 2^-969 <= |n| and
+ * |n / d| < 2^767.  For tests. */
+fg_status fg_check_shared_division(const double *num, const double *den, long n, long *mismatches);
 
 /* ---- tracing ---------------------------------------------------------- */
 /* Kernel timing of the fused path with HIP events on the context's
