@@ -40,6 +40,7 @@
 #define SRC64_TPB 512  // k_src: two 80 KB workgroups per CU, four waves per SIMD
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
+fg_status fg_disk_force_partial_async(fg_ctx *c, const double *dens, double x, double y, double rs, int *nblocks);
 
 
 __global__ void k_cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt, const double *invRmed,
@@ -1246,8 +1247,8 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
 //   - cell-planet separations are differenced in FP64;
 //   - the FARGO ring mean, Ntilde/Nshift and the uniform residual, the CFL
 //     count and the disk force are the FP64 code of the exact path.
-// The tile height th (rings per tile, <= SRC32_MAX_TH) is a launch
-// parameter chosen per grid size; the LDS tiles are carved from dynamic
+// The tile height th (rings per tile: 8, 16 or 32) is chosen per grid size
+// and dispatched to a template instance; the LDS tiles are carved from dynamic
 // shared memory: six (th + 6) x HC float arrays and two th x TW
 // increment arrays.
 #define SRC32_MAX_TH 32
@@ -1318,8 +1319,11 @@ __device__ __forceinline__ void own_load(const src32_args &a, int i0, int j0, in
 
 // the damping constants of the tile's rings are computed once per ring
 // into LDS
-__device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, int th, float *lds) {
-  const int hr = th + HALO_R + HALO_B;
+// ROWS (rings per tile) is a template parameter so that the LDS array offsets
+// and the per-thread trip counts are constants
+template <int ROWS>
+__device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, float *lds) {
+  constexpr int th = ROWS, hr = th + HALO_R + HALO_B;
   float(*sS)[HC] = (float(*)[HC])(lds);           // Sigma after the first damping
   float(*sA)[HC] = (float(*)[HC])(lds + hr * HC);  // pressure, then VradInt
   float(*sB)[HC] = (float(*)[HC])(lds + 2 * hr * HC);  // planet potential, then VthetaInt - vref
@@ -1585,9 +1589,18 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
 #undef GJ
 }
 
+// the tile heights fg_set_tiles accepts
+__device__ __forceinline__ void src32_tile_rows(const src32_args &a, int bx, int by, int th, float *lds) {
+  switch (th) {
+    case 8: src32_tile<8>(a, bx, by, lds); break;
+    case 32: src32_tile<32>(a, bx, by, lds); break;
+    default: src32_tile<16>(a, bx, by, lds); break;
+  }
+}
+
 __global__ __launch_bounds__(SRC32_TPB) void k_src32(src32_args a, int th) {
   extern __shared__ float lds_src32[];
-  src32_tile(a, blockIdx.x, blockIdx.y, th, lds_src32);
+  src32_tile_rows(a, blockIdx.x, blockIdx.y, th, lds_src32);
 }
 
 // Radial sweep in FP32 (k_trad above): Q of each ring is formed in FP64,
@@ -1806,6 +1819,50 @@ static fg_status fused_supported(fg_ctx *c) {
   return FG_OK;
 }
 
+// The host loop of DeviceReduce for the star force (dreduce_final: lanes 0
+// and 1 add the B block sums of x and y in order, then +0.0 for the inner
+// sum, SideEuler.cu:144) and, in waves 1-3 meanwhile, the CFL count of the
+// same substep (cfl_final: DT1D of every ring, Kernels.cu:1331-1337, and
+// the minimum with the ring minima, Kernels.cu:1337-1359; a minimum does
+// not depend on the order its candidates are compared in).
+__global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, int *cfl,
+                                                         double *dt1d, const double *vmoy, const double *newdt,
+                                                         const double *invRmed, double coef, double DeltaT,
+                                                         int nr) {
+  __shared__ double p[2 * FG_RED_BLOCKS];
+  __shared__ double wmin[4];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < 2 * B; k += 256) p[k] = partial[k];
+  __syncthreads();
+  if (tid < 2) {
+    double s = seq_sum_lds(p + tid * B, B);
+    out[tid] = 0.0 + s;
+  } else if (tid >= 64) {
+    double best = 1e300;
+    for (int i = tid - 64; i < nr; i += 192) {
+      if (i >= 1 && newdt[i] < best) best = newdt[i];
+      if (i < nr - 1) {
+        double d = coef / fabs(vmoy[i] * invRmed[i] - vmoy[i + 1] * invRmed[i + 1]);
+        dt1d[i] = d;
+        if (d < best) best = d;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(best, off, 64);
+      if (o < best) best = o;
+    }
+    if ((tid & 63) == 0) wmin[tid >> 6] = best;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double newdt_ = wmin[1];
+    if (wmin[2] < newdt_) newdt_ = wmin[2];
+    if (wmin[3] < newdt_) newdt_ = wmin[3];
+    if (DeltaT < newdt_) newdt_ = DeltaT;
+    cfl[0] = (int)(ceil(DeltaT / newdt_));
+  }
+}
+
 // results of the reductions for the next substep: CFL count and the disk
 // force on the star; copied to pinned memory behind an event
 // ring_partials: the star force comes from k_fin's per-ring compensated
@@ -1814,13 +1871,20 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
   if (ring_partials) {
     hipLaunchKernelGGL(k_force_rings, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out);
     FG_LAUNCH_CHECK();
-  } else if (fg_status s = fg_disk_force_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, c->red_out)) {
-    return s;
+    prof_mark(c);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
+                       c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+    FG_LAUNCH_CHECK();
+  } else {
+    // DeviceReduce partials, then one kernel for their final sum and the
+    // CFL count
+    int B = 0;
+    if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
+    prof_mark(c);
+    hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->red_out,
+                       c->cfl_out, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+    FG_LAUNCH_CHECK();
   }
-  prof_mark(c);
-  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
-                     c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
-  FG_LAUNCH_CHECK();
   prof_mark(c);
   FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -2081,7 +2145,7 @@ __global__ __launch_bounds__(SRC32_TPB) void k_batch_src(const batch_disk *__res
   extern __shared__ float lds_bsrc[];
   if ((int)blockIdx.x >= n) return;
   const int4 w = uniform4(items[blockIdx.x]);
-  if (D[w.x].active) src32_tile(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
+  if (D[w.x].active) src32_tile_rows(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
@@ -2229,7 +2293,7 @@ static fg_status batch_build(fg_batch_dev *b) {
   for (int k : order) {
     const fg_ctx *c = b->ctx[k];
     const int nr = c->nr, ns = c->ns, th = c->tile_src_th, band = c->tile_trad_band;
-    if (th < 1 || th > SRC32_MAX_TH || band < 1) {
+    if ((th != 8 && th != 16 && th != 32) || band < 1) {
       fg_set_error("batch: disk %d has tile shape th=%d band=%d", k, th, band);
       return FG_E_INVALID;
     }
@@ -2436,8 +2500,8 @@ void fg_batch_dev_invalidate(fg_batch_dev *b) {
 
 extern "C" fg_status fg_set_tiles(fg_ctx *c, int src_rows, int trad_band) {
   if (!c) return FG_E_INVALID;
-  if (src_rows < 1 || src_rows > SRC32_MAX_TH || trad_band < 1 || trad_band > 4096) {
-    fg_set_error("fg_set_tiles: rows %d (1..%d), band %d", src_rows, SRC32_MAX_TH, trad_band);
+  if ((src_rows != 8 && src_rows != 16 && src_rows != 32) || trad_band < 1 || trad_band > 4096) {
+    fg_set_error("fg_set_tiles: rows %d (8, 16 or 32), band %d (1..4096)", src_rows, trad_band);
     return FG_E_INVALID;
   }
   c->tile_src_th = src_rows;

@@ -218,8 +218,8 @@ fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domeg
 /* CFL count and star force of the current state (fused reductions) */
 fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
 fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);
-/* Tile shapes of the FP32 fused kernels: rings per source tile (1..32)
- * and rings per radial-sweep band.  Results do not depend on them; the
+/* Tile shapes of the fused kernels: rings per FP32 source tile (8, 16
+ * or 32) and rings per radial-sweep band.  Results do not depend on them; the
  * autotuner (cudaprotoplanetarydisks_amd/autotune.py) picks them per grid.
  * No reference counterpart (the reference's launch shape is fixed,
  * Main.cu:204-226). */
