@@ -17,8 +17,9 @@
 //   k_fin   one ring per workgroup: vrad recovery, third damping pass,
 //           temperature, and the next substep's CFL ring minimum
 //           X -> state A
-//   force   DeviceReduce-order disk force on the star (fg_reduce.hip) and
-//           the CFL count (k_cfl_final in fg_steps.hip)
+//   force   DeviceReduce-order disk force on the star (partials as in
+//           fg_reduce.hip) and the CFL count in one final kernel
+//           (k_force_cfl_final; FP32 mode: k_force_rings_cfl)
 //
 // Supported decks: locally isothermal, evanescent boundaries, no
 // self-gravity, per-DT CFL (everything else uses the per-routine path).
@@ -43,8 +44,6 @@ fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y,
 fg_status fg_disk_force_partial_async(fg_ctx *c, const double *dens, double x, double y, double rs, int *nblocks);
 
 
-__global__ void k_cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt, const double *invRmed,
-                            double coef, double DeltaT, int nr);
 
 struct fused_bufs {
   double *Bd, *Bvr, *Bvt;                // after k_src
@@ -976,8 +975,14 @@ __device__ __forceinline__ void force_rings_final(const double *fpart, int nr, d
   }
 }
 
-__global__ __launch_bounds__(256) void k_force_rings(const double *fpart, int nr, double *out2) {
+// FP32 mode: the star force from the per-ring partials, then the CFL count
+// (ConditionCFLKernel2D2 + 2D3, Kernels.cu:1331-1359) in the same launch
+__global__ __launch_bounds__(256) void k_force_rings_cfl(const double *fpart, int nr, double *out2, int *cfl,
+                                                         double *dt1d, const double *vmoy, const double *newdt,
+                                                         const double *invRmed, double coef, double DeltaT) {
   force_rings_final(fpart, nr, out2);
+  __syncthreads();
+  cfl_final(cfl, dt1d, vmoy, newdt, invRmed, coef, DeltaT, nr);
 }
 
 // one ring; ring holds NSEC doubles (the ring's final vtheta).  FIN_B
@@ -1869,11 +1874,9 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
 // partials (FP32 mode) instead of the DeviceReduce-order sum
 static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials = false) {
   if (ring_partials) {
-    hipLaunchKernelGGL(k_force_rings, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out);
-    FG_LAUNCH_CHECK();
-    prof_mark(c);
-    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
-                       c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+    prof_mark(c);  // (no separate force phase)
+    hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out,
+                       c->cfl_out, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
     FG_LAUNCH_CHECK();
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
