@@ -1594,18 +1594,12 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
 #undef GJ
 }
 
-// the tile heights fg_set_tiles accepts
-__device__ __forceinline__ void src32_tile_rows(const src32_args &a, int bx, int by, int th, float *lds) {
-  switch (th) {
-    case 8: src32_tile<8>(a, bx, by, lds); break;
-    case 32: src32_tile<32>(a, bx, by, lds); break;
-    default: src32_tile<16>(a, bx, by, lds); break;
-  }
-}
-
-__global__ __launch_bounds__(SRC32_TPB) void k_src32(src32_args a, int th) {
+// one kernel per tile height (8, 16 or 32, the heights fg_set_tiles
+// accepts): each instance needs only its own registers
+template <int ROWS>
+__global__ __launch_bounds__(SRC32_TPB) void k_src32(src32_args a) {
   extern __shared__ float lds_src32[];
-  src32_tile_rows(a, blockIdx.x, blockIdx.y, th, lds_src32);
+  src32_tile<ROWS>(a, blockIdx.x, blockIdx.y, lds_src32);
 }
 
 // Radial sweep in FP32 (k_trad above): Q of each ring is formed in FP64,
@@ -1994,10 +1988,15 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     s3.evanescent = sa.evanescent;
     s3.corotating = sa.corotating;
     const int th = c->tile_src_th;
-    FG_HIP(hipFuncSetAttribute((const void *)k_src32, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)src32_lds_bytes(SRC32_MAX_TH)));
-    hipLaunchKernelGGL(k_src32, dim3((ns + TW - 1) / TW, (nr + 1 + th - 1) / th), dim3(SRC32_TPB),
-                       src32_lds_bytes(th), c->stream, s3, th);
+    FG_HIP(hipFuncSetAttribute((const void *)k_src32<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)src32_lds_bytes(32)));
+    const dim3 g32((ns + TW - 1) / TW, (nr + 1 + th - 1) / th);
+    if (th == 8)
+      hipLaunchKernelGGL(k_src32<8>, g32, dim3(SRC32_TPB), src32_lds_bytes(8), c->stream, s3);
+    else if (th == 32)
+      hipLaunchKernelGGL(k_src32<32>, g32, dim3(SRC32_TPB), src32_lds_bytes(32), c->stream, s3);
+    else
+      hipLaunchKernelGGL(k_src32<16>, g32, dim3(SRC32_TPB), src32_lds_bytes(16), c->stream, s3);
   } else {
     hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC64_TPB), 0, c->stream,
                        sa);
@@ -2144,11 +2143,12 @@ __device__ __forceinline__ int2 uniform2(int2 w) {
 // one work item per workgroup: without a loop's carried scalars the tile's
 // uniform arguments fit the SGPRs instead of spilling into VGPR lanes
 // (253 -> 112 VGPRs)
+template <int ROWS>
 __global__ __launch_bounds__(SRC32_TPB) void k_batch_src(const batch_disk *__restrict__ D, const int4 *__restrict__ items, int n) {
   extern __shared__ float lds_bsrc[];
   if ((int)blockIdx.x >= n) return;
   const int4 w = uniform4(items[blockIdx.x]);
-  if (D[w.x].active) src32_tile_rows(D[w.x].src, w.y, w.z, w.w, lds_bsrc);
+  if (D[w.x].active) src32_tile<ROWS>(D[w.x].src, w.y, w.z, lds_bsrc);
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
@@ -2216,7 +2216,8 @@ struct fg_batch_dev {
   int4 *it_src;
   int2 *it_trad, *it_taz, *it_fin, *it_force;
   int n_src, n_trad, n_taz, n_fin, n_force;
-  int max_th, max_ns;
+  int n_src_rows[3];  // source items of tile height 8, 16, 32 (in that order in it_src)
+  int max_ns;
   hipGraph_t graph;
   hipGraphExec_t exec;
   int ready;  // items built and graph instantiated
@@ -2289,9 +2290,8 @@ static fg_status batch_build(fg_batch_dev *b) {
   std::vector<int> order(b->n);
   for (int k = 0; k < b->n; k++) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->ctx[x]->S > b->ctx[y]->S; });
-  std::vector<int4> src;
+  std::vector<int4> src_rows[3];  // by tile height 8, 16, 32
   std::vector<int2> trad, taz, fin, force;
-  b->max_th = 1;
   b->max_ns = 1;
   for (int k : order) {
     const fg_ctx *c = b->ctx[k];
@@ -2300,8 +2300,8 @@ static fg_status batch_build(fg_batch_dev *b) {
       fg_set_error("batch: disk %d has tile shape th=%d band=%d", k, th, band);
       return FG_E_INVALID;
     }
-    b->max_th = std::max(b->max_th, th);
     b->max_ns = std::max(b->max_ns, ns);
+    std::vector<int4> &src = src_rows[th == 8 ? 0 : th == 16 ? 1 : 2];
     for (int by = 0; by < (nr + 1 + th - 1) / th; by++)
       for (int bx = 0; bx < (ns + TW - 1) / TW; bx++) src.push_back(make_int4(k, bx, by, th));
     const int nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
@@ -2315,6 +2315,11 @@ static fg_status batch_build(fg_batch_dev *b) {
     const int B = red_blocks((long)nr * ns);
     for (int q = 0; q < B; q++) force.push_back(make_int2(k, q));
   }
+  std::vector<int4> src;
+  for (int h = 0; h < 3; h++) {
+    b->n_src_rows[h] = (int)src_rows[h].size();
+    src.insert(src.end(), src_rows[h].begin(), src_rows[h].end());
+  }
   b->n_src = (int)src.size();
   b->n_trad = (int)trad.size();
   b->n_taz = (int)taz.size();
@@ -2325,15 +2330,25 @@ static fg_status batch_build(fg_batch_dev *b) {
   if (fg_status s = upload_items(taz, &b->it_taz)) return s;
   if (fg_status s = upload_items(fin, &b->it_fin)) return s;
   if (fg_status s = upload_items(force, &b->it_force)) return s;
-  FG_HIP(hipFuncSetAttribute((const void *)k_batch_src, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)src32_lds_bytes(SRC32_MAX_TH)));
+  FG_HIP(hipFuncSetAttribute((const void *)k_batch_src<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)src32_lds_bytes(32)));
   // capture: arguments in, five phases, two reductions, results out
   hipStream_t st = b->stream;
   FG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   auto grid = [](int n) { return dim3(n < BATCH_CAP_BLOCKS ? (n > 0 ? n : 1) : BATCH_CAP_BLOCKS); };
   (void)hipMemcpyAsync(b->d_disk, b->h_disk, b->n * sizeof(batch_disk), hipMemcpyHostToDevice, st);
-  hipLaunchKernelGGL(k_batch_src, dim3(b->n_src > 0 ? b->n_src : 1), dim3(SRC32_TPB), src32_lds_bytes(b->max_th), st,
-                     b->d_disk, b->it_src, b->n_src);
+  // one source launch per tile height present (the work list holds them
+  // in order 8, 16, 32)
+  {
+    const int4 *it = b->it_src;
+    const int n8 = b->n_src_rows[0], n16 = b->n_src_rows[1], n32 = b->n_src_rows[2];
+    if (n8) hipLaunchKernelGGL(k_batch_src<8>, dim3(n8), dim3(SRC32_TPB), src32_lds_bytes(8), st, b->d_disk, it, n8);
+    if (n16)
+      hipLaunchKernelGGL(k_batch_src<16>, dim3(n16), dim3(SRC32_TPB), src32_lds_bytes(16), st, b->d_disk, it + n8, n16);
+    if (n32)
+      hipLaunchKernelGGL(k_batch_src<32>, dim3(n32), dim3(SRC32_TPB), src32_lds_bytes(32), st, b->d_disk,
+                         it + n8 + n16, n32);
+  }
   hipLaunchKernelGGL(k_batch_trad, grid(b->n_trad), dim3(TRAD_TPB), 0, st, b->d_disk, b->it_trad, b->n_trad);
   hipLaunchKernelGGL(k_batch_taz, grid(b->n_taz), dim3(64), 0, st, b->d_disk, b->it_taz, b->n_taz);
   hipLaunchKernelGGL(k_batch_fin, grid(b->n_fin), dim3(256), (size_t)b->max_ns * sizeof(double), st, b->d_disk,
