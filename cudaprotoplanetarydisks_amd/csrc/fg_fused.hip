@@ -1039,12 +1039,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       vt = damp_toward(vt, lam_i, vt0_i);
       s = damp_toward(s, lam_i, t.SigmaMed[i]);
     }
-    if (a.fpart) {  // gravity of the cell on the star (Kernels.cu:703-727 at x = y = 0)
-      const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
-      const double cm = surf * s;
-      kadd(fsx, fcx, cm * dx * inv3);
-      kadd(fsy, fcy, cm * dy * inv3);
-    }
     a.dens[l] = s;
     a.vr[l] = vr;
     a.vt[l] = vt;
@@ -1057,20 +1051,35 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     }
   }
   }
-  if (a.fpart) {
+  __syncthreads();
+  // FP32 mode: gravity of the ring's cells on the star (Kernels.cu:703-727
+  // at x = y = 0) from the densities just stored, summed by waves 1-3 while
+  // lane 0 forms the ring mean; cells j = tid - 64 + 192 u per thread, one
+  // compensated partial per thread, then per wave, merged in wave order
+  auto star_force = [&]() {
+    for (int j = tid - 64; j < ns; j += 192) {
+      const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
+      const double cm = surf * a.dens[base + j];
+      kadd(fsx, fcx, cm * dx * inv3);
+      kadd(fsy, fcy, cm * dy * inv3);
+    }
+  };
+  auto star_force_merge = [&]() {
     __shared__ double fw[4][4];
-    kwave(fsx, fcx);
-    kwave(fsy, fcy);
-    if ((tid & 63) == 0) {
-      fw[tid >> 6][0] = fsx;
-      fw[tid >> 6][1] = fcx;
-      fw[tid >> 6][2] = fsy;
-      fw[tid >> 6][3] = fcy;
+    if (tid >= 64) {
+      kwave(fsx, fcx);
+      kwave(fsy, fcy);
+      if ((tid & 63) == 0) {
+        fw[tid >> 6][0] = fsx;
+        fw[tid >> 6][1] = fcx;
+        fw[tid >> 6][2] = fsy;
+        fw[tid >> 6][3] = fcy;
+      }
     }
     __syncthreads();
-    if (tid == 64) {  // a lane of another wave than the ring sum below
-      double sx = fw[0][0], cx = fw[0][1], sy = fw[0][2], cy = fw[0][3];
-      for (int w = 1; w < 4; w++) {
+    if (tid == 64) {
+      double sx = fw[1][0], cx = fw[1][1], sy = fw[1][2], cy = fw[1][3];
+      for (int w = 2; w < 4; w++) {
         kmerge(sx, cx, fw[w][0], fw[w][1]);
         kmerge(sy, cy, fw[w][2], fw[w][3]);
       }
@@ -1079,8 +1088,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       a.fpart[4 * i + 2] = sy;
       a.fpart[4 * i + 3] = cy;
     }
-  }
-  __syncthreads();
+  };
   // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
   const double dxrad = t.Rsup[i] - t.Rinf[i];
   const double dxtheta = t.dxtheta_cfl[i];
@@ -1101,7 +1109,10 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       m /= (double)ns;
       a.vmoy[i] = m;
       s_vmoy = m;
-    } else if (tid >= 64 && i > 0) {
+    } else if (tid >= 64) {
+      if (a.fpart) star_force();
+    }
+    if (tid >= 64 && i > 0) {
       const double invdt1 = cs / mindx;
       const recip64 rdr(dxrad), rdt(dxtheta);
 #pragma unroll
@@ -1143,6 +1154,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         }
       }
     }
+    if (a.fpart) star_force_merge();
     __syncthreads();
     if (i == 0) return;
     if (tid >= 64) {
@@ -1172,7 +1184,10 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     m /= (double)ns;
     a.vmoy[i] = m;
     s_vmoy = m;
+  } else if (tid >= 64 && a.fpart) {
+    star_force();
   }
+  if (a.fpart) star_force_merge();
   __syncthreads();
   if (i == 0) return;
   const double vm = s_vmoy;
@@ -2171,7 +2186,7 @@ __global__ __launch_bounds__(64) void k_batch_taz(const batch_disk *__restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_fin(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_fin(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
