@@ -219,7 +219,11 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
   __syncthreads();
 
   // ---- S1: Substep1 (Kernels.cu:13-53) on rows [1, KHR), cols [1, KHC-1) ----
-  double regA[KPT], regB[KPT];
+  // The divisors of S1's pressure gradients, Sigma_i + Sigma_i-1 (radial)
+  // and Sigma_j + Sigma_j-1 (azimuthal), reappear halved in S3; x * 2 / d is
+  // x / (d / 2) (both scalings exact), so S1 divides by the halved sums and
+  // keeps their reciprocal refinements (recip64) for S3 in the same thread.
+  double regA[KPT], regB[KPT], hrad[KPT], hazi[KPT];
   {
 #pragma unroll
     for (int k = 0; k < KPT; k++) {
@@ -228,9 +232,12 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
       int r = q / KHC, cc = q - r * KHC;
       int i = GI(r);
       double vri = 0.0, vti = 0.0;
+      hrad[k] = hazi[k] = 0.0;
       if (r >= 1 && cc >= 1 && cc < KHC - 1 && i >= 0 && i < nr) {
         if (i > 0) {
-          double gradp = (sA[r][cc] - sA[r - 1][cc]) * 2.0 / (sS[r][cc] + sS[r - 1][cc]) * rt[RT_IDMED][r];
+          const recip64 hd(0.5 * (sS[r][cc] + sS[r - 1][cc]));
+          hrad[k] = hd.r;
+          double gradp = hd.div(sA[r][cc] - sA[r - 1][cc]) * rt[RT_IDMED][r];
           double gradphi = (sB[r][cc] - sB[r - 1][cc]) * rt[RT_IDMED][r];
           double vt2 = sC[r][cc] + sC[r][cc + 1] + sC[r - 1][cc] + sC[r - 1][cc + 1];
           vt2 = vt2 / 4.0 + a.omega_frame * rt[RT_INF][r];
@@ -240,7 +247,9 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
           vri = sD[r][cc] + dt * vradint2;
         }
         const double invdxtheta = rt[RT_IDXT][r];
-        double gradp = (sA[r][cc] - sA[r][cc - 1]) * 2.0 / (sS[r][cc] + sS[r][cc - 1]) * invdxtheta;
+        const recip64 hd(0.5 * (sS[r][cc] + sS[r][cc - 1]));
+        hazi[k] = hd.r;
+        double gradp = hd.div(sA[r][cc] - sA[r][cc - 1]) * invdxtheta;
         double gradphi = (sB[r][cc] - sB[r][cc - 1]) * invdxtheta;
         double v = sC[r][cc] - dt * (gradp + gradphi);
         v += dt * rt[RT_SUPP][r];
@@ -303,15 +312,17 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
     int r = q / KHC, cc = q - r * KHC;
     int i = GI(r);
     if (r < 2 || r >= KHR - 1 || cc < 3 || cc >= KHC - 3 || i < 1 || i >= nr) continue;
+    // (S1 covered this cell with i > 0: both halved sums' refinements are in
+    // hazi / hrad)
     if (i < nr - 1 && r < KHR - 2)
-      sB[r][cc] += dt * rt[RT_IMED][r] *
-                   ((rt[RT_SUP][r] * sE[r + 1][cc] - rt[RT_INF][r] * sE[r][cc]) * rt[RT_IDSUP][r] +
-                    (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5 * (sE[r][cc] + sE[r + 1][cc])) /
-                   (0.5 * (sS[r][cc] + sS[r][cc - 1]));
-    sA[r][cc] += dt * rt[RT_IINF][r] *
-                 ((rt[RT_MED][r] * sC[r][cc] - rt[RT_MED][r - 1] * sC[r - 1][cc]) * rt[RT_IDMED][r] +
-                  (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5 * (sD[r][cc] + sD[r - 1][cc])) /
-                 (0.5 * (sS[r][cc] + sS[r - 1][cc]));
+      sB[r][cc] += recip64(0.5 * (sS[r][cc] + sS[r][cc - 1]), hazi[kq])
+                       .div(dt * rt[RT_IMED][r] *
+                            ((rt[RT_SUP][r] * sE[r + 1][cc] - rt[RT_INF][r] * sE[r][cc]) * rt[RT_IDSUP][r] +
+                             (sD[r][cc] - sD[r][cc - 1]) * invdphi + 0.5 * (sE[r][cc] + sE[r + 1][cc])));
+    sA[r][cc] += recip64(0.5 * (sS[r][cc] + sS[r - 1][cc]), hrad[kq])
+                     .div(dt * rt[RT_IINF][r] *
+                          ((rt[RT_MED][r] * sC[r][cc] - rt[RT_MED][r - 1] * sC[r - 1][cc]) * rt[RT_IDMED][r] +
+                           (sE[r][cc + 1] - sE[r][cc]) * invdphi - 0.5 * (sD[r][cc] + sD[r - 1][cc])));
   }
   __syncthreads();
 
