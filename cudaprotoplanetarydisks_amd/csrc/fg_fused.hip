@@ -988,7 +988,7 @@ __device__ __forceinline__ void force_rings_final(const double *fpart, int nr, d
 
 // FP32 mode: the star force from the per-ring partials, then the CFL count
 // (ConditionCFLKernel2D2 + 2D3, Kernels.cu:1331-1359) in the same launch
-__global__ __launch_bounds__(256) void k_force_rings_cfl(const double *fpart, int nr, double *out2, int *cfl,
+__global__ __launch_bounds__(256) void k_force_rings_cfl(const double *fpart, int nr, double *out2, double *cfl,
                                                          double *dt1d, const double *vmoy, const double *newdt,
                                                          const double *invRmed, double coef, double DeltaT) {
   force_rings_final(fpart, nr, out2);
@@ -1850,7 +1850,7 @@ static fg_status fused_supported(fg_ctx *c) {
 // same substep (cfl_final: DT1D of every ring, Kernels.cu:1331-1337, and
 // the minimum with the ring minima, Kernels.cu:1337-1359; a minimum does
 // not depend on the order its candidates are compared in).
-__global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, int *cfl,
+__global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, double *cfl,
                                                          double *dt1d, const double *vmoy, const double *newdt,
                                                          const double *invRmed, double coef, double DeltaT,
                                                          int nr) {
@@ -1884,9 +1884,12 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
     if (wmin[2] < newdt_) newdt_ = wmin[2];
     if (wmin[3] < newdt_) newdt_ = wmin[3];
     if (DeltaT < newdt_) newdt_ = DeltaT;
-    cfl[0] = (int)(ceil(DeltaT / newdt_));
+    cfl[0] = (double)(int)(ceil(DeltaT / newdt_));
   }
 }
+
+// slot of red_out holding the fused path's CFL count (as a double)
+#define FUSED_CFL_SLOT 6
 
 // results of the reductions for the next substep: CFL count and the disk
 // force on the star; copied to pinned memory behind an event
@@ -1896,7 +1899,7 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
   if (ring_partials) {
     prof_mark(c);  // (no separate force phase)
     hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out,
-                       c->cfl_out, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
+                       c->red_out + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
     FG_LAUNCH_CHECK();
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
@@ -1905,12 +1908,13 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
     if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
     prof_mark(c);
     hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->red_out,
-                       c->cfl_out, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+                       c->red_out + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
     FG_LAUNCH_CHECK();
   }
   prof_mark(c);
-  FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // one copy for the force (slots 0-1) and the count (slot FUSED_CFL_SLOT)
+  FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, (FUSED_CFL_SLOT + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
   return FG_OK;
 }
 
@@ -1934,7 +1938,7 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   if (!c) return FG_E_INVALID;
   FG_HIP(hipStreamSynchronize(c->stream));
   prof_fold(c);
-  if (ncfl) *ncfl = c->h_cfl[0];
+  if (ncfl) *ncfl = (int)c->h_out[FUSED_CFL_SLOT];
   if (force2) {
     force2[0] = c->h_out[0];
     force2[1] = c->h_out[1];

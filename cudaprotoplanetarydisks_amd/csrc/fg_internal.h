@@ -83,7 +83,8 @@ struct fg_ctx {
   double *LostByDisk;  // ns
   // reduction scratch
   double *red_partial;  // 2 * FG_RED_BLOCKS
-  double *red_out;      // [0..1] force, [2] lost mass, [3] circumplanetary mass, [4..5] ring means
+  double *red_out;      // [0..1] force, [2] lost mass, [3] circumplanetary mass, [4..5] ring means,
+                        // [6] fused path's CFL count
   int *cfl_out;         // [0] CFL count
   double *h_out;        // pinned host mirror of red_out (4 doubles) + CFL
   int *h_cfl;

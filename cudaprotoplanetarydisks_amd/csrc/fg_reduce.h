@@ -279,7 +279,9 @@ __device__ __forceinline__ void dreduce_final(const double *partial, int B, doub
 // ConditionCFLKernel2D3 (Kernels.cu:1337-1360) and the host side of
 // ConditionCFL (SourceEuler.cu:520-546): min of the ring minima and the
 // 1D shear limits, then the sub-cycle count ceil(DeltaT / dt)
-__device__ __forceinline__ void cfl_final(int *cfl, double *dt1d, const double *vmoy, const double *newdt,
+// cfl: an int, or a double slot of a result block copied to the host at once
+template <class T>
+__device__ __forceinline__ void cfl_final(T *cfl, double *dt1d, const double *vmoy, const double *newdt,
                                           const double *invRmed, double coef, double DeltaT, int nr) {
   __shared__ double red[256];
   double best = 1e300;
@@ -300,6 +302,6 @@ __device__ __forceinline__ void cfl_final(int *cfl, double *dt1d, const double *
   if (threadIdx.x == 0) {
     double newdt_ = red[0];
     if (DeltaT < newdt_) newdt_ = DeltaT;
-    cfl[0] = (int)(ceil(DeltaT / newdt_));
+    cfl[0] = (T)(int)(ceil(DeltaT / newdt_));
   }
 }
