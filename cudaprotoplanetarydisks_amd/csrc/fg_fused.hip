@@ -783,7 +783,9 @@ __device__ __forceinline__ void wave_sweep32(double (&q)[NQ][CL], const float (&
 template <int CL, bool F32>
 __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   constexpr int N = 64 * CL, P = N - TAZW_HL - TAZW_HR;
-  __shared__ double stage[N];
+  // LDS: the load stage, then the four output stages (row 0 reused)
+  __shared__ double ostage[4][N];
+  double *stage = ostage[0];
   const int ns = a.ns, l = threadIdx.x;
   const int g0 = chunk * P - TAZW_HL;  // unwrapped index of the chunk's first cell
   const fg_dev_tables &t = a.t;
@@ -868,23 +870,29 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   // that lie in [0, NSEC)
   int sh = nsh % ns;
   if (sh < 0) sh += ns;
+  // all four outputs staged at once, then their coalesced stores back to
+  // back (one LDS round trip instead of four)
 #pragma unroll
-  for (int o = 0; o < 4; o++) {
-    double *dst = o == 0 ? a.xd : o == 1 ? a.xvt : o == 2 ? a.xrmp : a.xrmm;
+  for (int c = 0; c < CL; c++) {
+    ostage[0][l * CL + c] = q[NQ - 1][c];
+    ostage[1][l * CL + c] = vt[c];
+    ostage[2][l * CL + c] = q[0][c];
+    ostage[3][l * CL + c] = q[1][c];
+  }
+  __syncthreads();
 #pragma unroll
-    for (int c = 0; c < CL; c++) stage[l * CL + c] = o == 0 ? q[NQ - 1][c] : o == 1 ? vt[c] : q[o - 2][c];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < CL; m++) {
-      const int k = l + 64 * m;
-      const int g = g0 + k;
-      if (k >= TAZW_HL && k < TAZW_HL + P && g < ns) {
-        int jo = g + sh;
-        if (jo >= ns) jo -= ns;
-        dst[base + jo] = stage[k];
-      }
+  for (int m = 0; m < CL; m++) {
+    const int k = l + 64 * m;
+    const int g = g0 + k;
+    if (k >= TAZW_HL && k < TAZW_HL + P && g < ns) {
+      int jo = g + sh;
+      if (jo >= ns) jo -= ns;
+      const long lo = base + jo;
+      a.xd[lo] = ostage[0][k];
+      a.xvt[lo] = ostage[1][k];
+      a.xrmp[lo] = ostage[2][k];
+      a.xrmm[lo] = ostage[3][k];
     }
-    __syncthreads();
   }
 }
 
