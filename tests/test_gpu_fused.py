@@ -51,6 +51,14 @@ def test_fused_cases_bitexact(deck, case):
     run_fused(deck(**case), 15)
 
 
+@pytest.mark.parametrize("case", [
+    dict(Nrad=12, Nsec=3200),  # k_fin's ring pass without the overlapped CFL split (NSEC > 3072)
+    dict(Nrad=70, Nsec=4096),  # the widest ring the fused path takes
+], ids=["nsec3200", "nsec4096"])
+def test_fused_wide_rings_bitexact(deck, case):
+    run_fused(deck(**case), 4)
+
+
 def test_fused_planets_feel_disk(deck, tmp_path):
     cfg = write_planets(tmp_path / "p.cfg", [("A", 1.0, 1e-3, 0.0, "YES", "NO"), ("B", 1.7, 5e-4, 0.0, "YES", "YES")])
     run_fused(deck(Nrad=32, Nsec=96, PlanetConfig=cfg), 15)
