@@ -39,6 +39,15 @@ def test_fp32_odd_grid_and_corotating(deck):
         assert e["vtheta"]["linf"] <= VEL_LINF, rep
 
 
+@pytest.mark.parametrize("case", [dict(Nrad=16, Nsec=3200), dict(Nrad=24, Nsec=4096)], ids=["nsec3200", "nsec4096"])
+def test_fp32_wide_rings(deck, case):
+    # k_fin's ring pass past the overlapped CFL split, and the widest ring
+    for rep in compare(deck(**case), 4, every=2):
+        e = rep["errors"]
+        assert e["dens"]["linf"] <= DENS_LINF, rep
+        assert e["vtheta"]["linf"] <= VEL_LINF, rep
+
+
 def test_fp32_same_fargo_shift_and_cfl(deck):
     d = deck()
     s64 = Simulation(d, fused=True, precision="fp64")
