@@ -57,6 +57,16 @@ def test_tiles_do_not_change_results(tmp_path):
             assert np.array_equal(a, b)
 
 
+def test_tile_heights_other_than_8_16_32_are_rejected(tmp_path):
+    from cudaprotoplanetarydisks_amd.native import FargoError
+    ds = decks(tmp_path, SMALL[:1])
+    with Ensemble(ds, tiles={}) as ens:
+        for rows in (1, 12, 24, 64):
+            with pytest.raises(FargoError):
+                ens.set_tiles(0, rows, 8)
+        ens.set_tiles(0, 32, 8)
+
+
 def test_disks_stop_at_the_first_dt_boundary_after_target(tmp_path):
     ds = decks(tmp_path, SMALL)
     DT = 0.314159265359
