@@ -15,8 +15,8 @@ __global__ void k_dreduce_final(const double *partial, int B, double *out, int t
 template <class Src>
 static fg_status dreduce(fg_ctx *c, Src src, long n, double *out_dev, int two, int add_inner) {
   int B = red_blocks(n);
-  // 16 summand rows per round: 24 or 48 measured the same or slower
-  // (fewer resident workgroups)
+  // 16 summand rows per round: 8, 24 or 48 measured slower (more rounds, or
+  // fewer resident workgroups)
   hipLaunchKernelGGL((k_dreduce_partial<Src, RED_CH>), dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B,
                      c->red_partial);
   FG_LAUNCH_CHECK();
