@@ -246,6 +246,7 @@ extern "C" fg_status fg_destroy(fg_ctx *c) {
   for (int k = 0; k < 4; k++)
     if (c->fu_mom[k]) (void)hipFree(c->fu_mom[k]);
   if (c->h_out) (void)hipHostFree(c->h_out);
+  if (c->h_fused) (void)hipHostFree(c->h_fused);
   if (c->h_cfl) (void)hipHostFree(c->h_cfl);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

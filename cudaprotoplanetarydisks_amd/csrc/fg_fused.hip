@@ -1816,6 +1816,8 @@ static fg_status fused_alloc(fg_ctx *c) {
   c->fu_ring = pool;
   FG_HIP(hipMalloc((void **)&c->fu_fpart, 4 * (size_t)(c->nr + 1) * sizeof(double)));
   FG_HIP(hipMemset(c->fu_fpart, 0, 4 * (size_t)(c->nr + 1) * sizeof(double)));
+  FG_HIP(hipHostMalloc((void **)&c->h_fused, 8 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
+  FG_HIP(hipHostGetDevicePointer((void **)&c->d_fused, c->h_fused, 0));
   return FG_OK;
 }
 
@@ -1896,7 +1898,7 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
   }
 }
 
-// slot of red_out holding the fused path's CFL count (as a double)
+// slot of the fused result block (h_fused) holding the CFL count (as a double)
 #define FUSED_CFL_SLOT 6
 
 // results of the reductions for the next substep: CFL count and the disk
@@ -1906,8 +1908,8 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
 static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials = false) {
   if (ring_partials) {
     prof_mark(c);  // (no separate force phase)
-    hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->red_out,
-                       c->red_out + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
+    hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->d_fused,
+                       c->d_fused + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
     FG_LAUNCH_CHECK();
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
@@ -1915,14 +1917,11 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
     int B = 0;
     if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
     prof_mark(c);
-    hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->red_out,
-                       c->red_out + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+    hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->d_fused,
+                       c->d_fused + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
     FG_LAUNCH_CHECK();
   }
   prof_mark(c);
-  // one copy for the force (slots 0-1) and the count (slot FUSED_CFL_SLOT)
-  FG_HIP(hipMemcpyAsync(c->h_out, c->red_out, (FUSED_CFL_SLOT + 1) * sizeof(double), hipMemcpyDeviceToHost,
-                        c->stream));
   return FG_OK;
 }
 
@@ -1946,10 +1945,10 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   if (!c) return FG_E_INVALID;
   FG_HIP(hipStreamSynchronize(c->stream));
   prof_fold(c);
-  if (ncfl) *ncfl = (int)c->h_out[FUSED_CFL_SLOT];
+  if (ncfl) *ncfl = (int)c->h_fused[FUSED_CFL_SLOT];
   if (force2) {
-    force2[0] = c->h_out[0];
-    force2[1] = c->h_out[1];
+    force2[0] = c->h_fused[0];
+    force2[1] = c->h_fused[1];
   }
   if (lost) *lost = 0.0;  // evanescent edges: no outflow is recorded
   return FG_OK;

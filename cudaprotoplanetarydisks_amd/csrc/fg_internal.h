@@ -111,6 +111,9 @@ struct fg_ctx {
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results
   double *fu_fpart;   // FP32 mode: per-ring star-force partials (4 * NRAD)
+  // fused path's force and CFL count: written by the final reduction kernel
+  // straight into coherent pinned host memory (no copy); device alias
+  double *h_fused, *d_fused;
   // Hill-cut force variants (fg_disk_force_hill), allocated on first use
   double *hill_partial;  // 4 * FG_HILL_MAX * FG_RED_BLOCKS block sums
   double *hill_out;      // 4 * FG_HILL_MAX results
