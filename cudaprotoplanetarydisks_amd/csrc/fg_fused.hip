@@ -1007,10 +1007,29 @@ __global__ __launch_bounds__(256) void k_force_rings_cfl(const double *fpart, in
 // one ring; ring holds NSEC doubles (the ring's final vtheta).  FIN_B
 // cells per thread have their loads issued together (4 for the single-disk
 // kernel; 1 in the batched one, which would otherwise lose a wave per SIMD)
+// minimum over the workgroup's 256 threads into *out (thread 0): within
+// each wave by shuffles, then the four wave minima (a minimum does not
+// depend on the order of its comparisons)
+__device__ __forceinline__ void ring_min_store(double best, double *red, double *out) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(best, off, 64);
+    if (o < best) best = o;
+  }
+  const int tid = threadIdx.x;
+  if ((tid & 63) == 0) red[tid >> 6] = best;
+  __syncthreads();
+  if (tid == 0) {
+    double m = red[0];
+    for (int w = 1; w < 4; w++)
+      if (red[w] < m) m = red[w];
+    *out = m;
+  }
+}
+
 #define FIN_OV_K 16  // cells per thread of the overlapped CFL pass (rings up to 3072 cells)
 template <int FIN_B>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   __shared__ double s_vmoy;
   const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
   const fg_dev_tables &t = a.t;
@@ -1189,13 +1208,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         if (dtc < best) best = dtc;
       }
     }
-    red[tid] = best;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) a.newdt[i] = red[0];
+    ring_min_store(best, red, a.newdt + i);
     return;
   }
   if (tid == 0) {
@@ -1250,13 +1263,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     if (dtc < best) best = dtc;
   }
   }
-  red[tid] = best;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s && red[tid + s] < red[tid]) red[tid] = red[tid + s];
-    __syncthreads();
-  }
-  if (tid == 0) a.newdt[i] = red[0];
+  ring_min_store(best, red, a.newdt + i);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Each XCD gets a
