@@ -276,7 +276,10 @@ extern "C" fg_status fg_upload_state(fg_ctx *c, const double *dens, const double
   if ((st = put(c, c->f[FG_VRAD], vrad))) return st;
   if ((st = put(c, c->f[FG_VTHETA], vtheta))) return st;
   if ((st = put(c, c->f[FG_ENERGY], energy))) return st;
+  // a new state: crash flags raised by passes over the old one no longer apply
+  FG_HIP(hipMemsetAsync(c->cfl_out + 1, 0, sizeof(int), c->stream));
   FG_HIP(hipStreamSynchronize(c->stream));
+  if (c->h_fused) *(volatile int *)(c->h_fused + FG_FUSED_CRASH_SLOT) = 0;
   return FG_OK;
 }
 

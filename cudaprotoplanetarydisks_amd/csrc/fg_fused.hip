@@ -934,6 +934,7 @@ struct fin_args {
   double *dens, *vr, *vt, *temp;
   double *vmoy, *newdt;
   double *fpart;  // FP32 mode: per-ring compensated star-force partials (4 per ring), else NULL
+  int *crash;     // set to 1 when a stored density is negative or not finite
   const double *nu;
   fg_dev_tables t;
   double dt, omega_frame;
@@ -1046,6 +1047,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   // sums the reference's per-cell expression in DeviceReduce order)
   const double inv3 = 1.0 / (Rm * Rm) / Rm;
   const bool radial = i > 0 && i < nr;  // vrad recovery reads rings i-1 and i
+  bool bad = false;                      // a density below 0, infinite or NaN
   // FIN_B cells per thread per batch: all their loads first, then the math
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B];
@@ -1078,6 +1080,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       s = damp_toward(s, lam_i, t.SigmaMed[i]);
     }
     a.dens[l] = s;
+    bad |= !(s >= 0.0 && s < HUGE_VAL);
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
@@ -1089,6 +1092,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     }
   }
   }
+  if (bad) *a.crash = 1;
   __syncthreads();
   // FP32 mode: gravity of the ring's cells on the star (Kernels.cu:703-727
   // at x = y = 0) from the densities just stored, summed by waves 1-3 while
@@ -1813,18 +1817,36 @@ extern "C" int fg_profile_read(fg_ctx *c, double *ms, long *calls, int max) {
   }
   return n;
 }
+// The fused path's buffers are published together, only once every
+// allocation succeeded (a failed attempt frees what it got and a later call
+// starts again).
 static fg_status fused_alloc(fg_ctx *c) {
   if (c->fu_ring) return FG_OK;
   const size_t S = (size_t)c->S;
+  const size_t fpart_bytes = 4 * (size_t)(c->nr + 1) * sizeof(double);
   // 14 arrays: B (3), R (5), X (4), pressure, spare
-  double *pool = nullptr;
-  FG_HIP(hipMalloc((void **)&pool, 14 * S * sizeof(double)));
-  FG_HIP(hipMemset(pool, 0, 14 * S * sizeof(double)));
+  double *pool = nullptr, *fpart = nullptr, *h = nullptr, *d = nullptr;
+  hipError_t e = hipMalloc((void **)&pool, 14 * S * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(pool, 0, 14 * S * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&fpart, fpart_bytes);
+  if (e == hipSuccess) e = hipMemset(fpart, 0, fpart_bytes);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&h, FG_FUSED_SLOTS * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped);
+  if (e == hipSuccess) {
+    memset(h, 0, FG_FUSED_SLOTS * sizeof(double));
+    e = hipHostGetDevicePointer((void **)&d, h, 0);
+  }
+  if (e != hipSuccess) {
+    if (pool) (void)hipFree(pool);
+    if (fpart) (void)hipFree(fpart);
+    if (h) (void)hipHostFree(h);
+    fg_set_error("fused path buffers: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? FG_E_NOMEM : FG_E_HIP;
+  }
+  c->fu_fpart = fpart;
+  c->h_fused = h;
+  c->d_fused = d;
   c->fu_ring = pool;
-  FG_HIP(hipMalloc((void **)&c->fu_fpart, 4 * (size_t)(c->nr + 1) * sizeof(double)));
-  FG_HIP(hipMemset(c->fu_fpart, 0, 4 * (size_t)(c->nr + 1) * sizeof(double)));
-  FG_HIP(hipHostMalloc((void **)&c->h_fused, 8 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
-  FG_HIP(hipHostGetDevicePointer((void **)&c->d_fused, c->h_fused, 0));
   return FG_OK;
 }
 
@@ -1905,9 +1927,6 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
   }
 }
 
-// slot of the fused result block (h_fused) holding the CFL count (as a double)
-#define FUSED_CFL_SLOT 6
-
 // results of the reductions for the next substep: CFL count and the disk
 // force on the star; copied to pinned memory behind an event
 // ring_partials: the star force comes from k_fin's per-ring compensated
@@ -1916,7 +1935,7 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
   if (ring_partials) {
     prof_mark(c);  // (no separate force phase)
     hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->d_fused,
-                       c->d_fused + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
+                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
     FG_LAUNCH_CHECK();
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
@@ -1925,7 +1944,7 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
     if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
     prof_mark(c);
     hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->d_fused,
-                       c->d_fused + FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
     FG_LAUNCH_CHECK();
   }
   prof_mark(c);
@@ -1935,7 +1954,8 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
 // CFL ring pass over the current state (first substep, or after the
 // per-routine path touched the state)
 __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const double *vt, const double *cs,
-                           fg_dev_tables t, int nr, int ns, int fast);
+                           const double *dens, const double *energy, int *crash, fg_dev_tables t, int nr, int ns,
+                           int fast);
 
 extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
   if (!c) return FG_E_INVALID;
@@ -1943,19 +1963,41 @@ extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
   if (fg_status s = fused_alloc(c)) return s;
   size_t shm = (size_t)(c->ns + 256) * sizeof(double);
   hipLaunchKernelGGL(k_cfl_ring, dim3(c->nr), dim3(256), shm, c->stream, c->Vmoy, c->newDT, c->f[FG_VRAD],
-                     c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->t, c->nr, c->ns, c->p.fast_transport);
+                     c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->f[FG_DENS], (const double *)nullptr,
+                     (int *)(c->d_fused + FG_FUSED_CRASH_SLOT), c->t, c->nr, c->ns, c->p.fast_transport);
   FG_LAUNCH_CHECK();
   return fused_reductions(c, DeltaT);
 }
 
+// Results of the last fused reductions, checked: FG_E_CRASH when a pass
+// flagged a negative or non-finite density, or when the CFL count or the
+// force shows a time step that went to 0, infinity or NaN (the flag is
+// cleared once reported; the next pass over a bad state sets it again).
+fg_status fg_fused_check(fg_ctx *c, double cfl, const double *force2) {
+  volatile int *flag = (volatile int *)(c->h_fused + FG_FUSED_CRASH_SLOT);
+  if (*flag) {
+    *flag = 0;
+    fg_set_error("crash: negative or non-finite density");
+    return FG_E_CRASH;
+  }
+  return fg_check_substep_results(cfl, force2);
+}
+
 extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, double *lost) {
   if (!c) return FG_E_INVALID;
+  if (!c->h_fused) {
+    fg_set_error("fg_fused_results: no fused reductions were launched (fg_fused_prepare or fg_step_fused first)");
+    return FG_E_INVALID;
+  }
   FG_HIP(hipStreamSynchronize(c->stream));
   prof_fold(c);
-  if (ncfl) *ncfl = (int)c->h_fused[FUSED_CFL_SLOT];
+  const double cfl = c->h_fused[FG_FUSED_CFL_SLOT];
+  const double f[2] = {c->h_fused[0], c->h_fused[1]};
+  if (fg_status s = fg_fused_check(c, cfl, f)) return s;
+  if (ncfl) *ncfl = (int)cfl;
   if (force2) {
-    force2[0] = c->h_fused[0];
-    force2[1] = c->h_fused[1];
+    force2[0] = f[0];
+    force2[1] = f[1];
   }
   if (lost) *lost = 0.0;  // evanescent edges: no outflow is recorded
   return FG_OK;
@@ -2129,6 +2171,7 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
   fa.fpart = fp32 ? c->fu_fpart : nullptr;
+  fa.crash = (int *)(c->d_fused + FG_FUSED_CRASH_SLOT);
   prof_mark(c);
   hipLaunchKernelGGL(k_fin, dim3(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD)), dim3(256), (size_t)ns * sizeof(double),
                      c->stream, fa);
@@ -2510,6 +2553,7 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fa.evanescent = c->p.evanescent;
   fa.fast = c->p.fast_transport;
   fa.fpart = c->fu_fpart;
+  fa.crash = (int *)(c->d_fused + FG_FUSED_CRASH_SLOT);
   d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
   d.ncell = (long)nr * ns;
   d.B = red_blocks(d.ncell);

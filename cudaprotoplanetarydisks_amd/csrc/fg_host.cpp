@@ -295,6 +295,10 @@ struct fg_sim {
   long nsub = 0;          // substeps taken since creation
   int planets_fresh = 0;  // planetK.dat emptied (first output) or kept (restart)
   int results_ready = 0;  // fused CFL count / star force are current
+  // bumped by every change of the state outside a batch launch (set_state,
+  // restart, a single-disk substep); a batch re-derives the CFL count and
+  // star force of a disk whose generation it has not seen
+  unsigned long state_gen = 0;
 };
 
 namespace {
@@ -1073,6 +1077,7 @@ static void fused_close(fg_sim *s, const fused_launch &L, int *closed) {
 static fg_status fused_substep(fg_sim *s, int *closed) {
   const Deck &d = s->d;
   fg_ctx *c = s->ctx;
+  s->state_gen++;
   fused_open(s);
   if (!s->results_ready) {
     CHECK(fg_fused_prepare(c, d.DT - s->dtemp));
@@ -1094,6 +1099,7 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
 extern "C" fg_status fg_sim_substep(fg_sim *s, int *closed) {
   if (!s) return FG_E_INVALID;
   if (s->fused_active) return fused_substep(s, closed);
+  s->state_gen++;
   const Deck &d = s->d;
   fg_ctx *c = s->ctx;
   if (!s->in_dt) CHECK(begin_dt(s));
@@ -1372,6 +1378,7 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
   s->LostMass = lost;
   s->in_dt = 0;
   s->results_ready = 0;
+  s->state_gen++;
   s->planets_fresh = 1;  // keep the planet files: their lines up to idx are this run's history
   CHECK(fg_upload_state(s->ctx, dens.data(), vr.data(), vt.data(), d.Adiabatic ? en.data() : nullptr));
   CHECK(fg_sound_speed(s->ctx));
@@ -1418,6 +1425,7 @@ extern "C" fg_status fg_sim_set_state(fg_sim *s, const double *dens, const doubl
                                       const double *energy) {
   if (!s) return FG_E_INVALID;
   s->results_ready = 0;
+  s->state_gen++;
   return fg_upload_state(s->ctx, dens, vrad, vtheta, energy);
 }
 
@@ -1498,6 +1506,7 @@ fg_status fg_batch_dev_launch(fg_batch_dev *b);
 fg_status fg_batch_dev_wait(fg_batch_dev *b);
 void fg_batch_dev_result(const fg_batch_dev *b, int k, int *cfl, double *force2);
 void fg_batch_dev_invalidate(fg_batch_dev *b);
+fg_status fg_fused_check(fg_ctx *c, double cfl, const double *force2);
 
 struct fg_batch {
   std::vector<fg_sim *> sims;
@@ -1506,6 +1515,7 @@ struct fg_batch {
   std::vector<double> force;
   std::vector<fused_launch> L;
   std::vector<char> active;
+  std::vector<unsigned long> seen_gen;  // state generation of each disk at its last batch step
   int started = 0;
 };
 
@@ -1552,6 +1562,7 @@ extern "C" fg_status fg_batch_create(int n, const char *const *parfiles, const f
   b->force.assign(2 * (size_t)n, 0.0);
   b->L.resize(n);
   b->active.assign(n, 0);
+  b->seen_gen.assign(n, 0);
   *out = b;
   return FG_OK;
 }
@@ -1566,15 +1577,23 @@ extern "C" fg_status fg_batch_set_tiles(fg_batch *b, int k, int src_rows, int tr
 extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) {
   if (!b) return FG_E_INVALID;
   const int n = (int)b->sims.size();
-  if (!b->started) {  // CFL count and star force of every initial state
-    for (int k = 0; k < n; k++) {
-      fg_sim *s = b->sims[k];
-      fused_open(s);
-      CHECK(fg_fused_prepare(s->ctx, s->d.DT - s->dtemp));
-      CHECK(fg_fused_results(s->ctx, &b->cfl[k], &b->force[2 * k], nullptr));
+  // CFL count and star force of every initial state, and again for a disk
+  // whose state changed since its last batch step (fg_sim_set_state,
+  // fg_sim_restart or single-disk substeps through fg_batch_sim)
+  for (int k = 0; k < n; k++) {
+    fg_sim *s = b->sims[k];
+    if (b->started && s->state_gen == b->seen_gen[k]) continue;
+    fused_open(s);
+    CHECK(fg_fused_prepare(s->ctx, s->d.DT - s->dtemp));
+    fg_status st = fg_fused_results(s->ctx, &b->cfl[k], &b->force[2 * k], nullptr);
+    if (st) {
+      const std::string why = fg_last_error();  // the message buffer is rewritten below
+      host_error("batch disk %d: %s", k, why.c_str());
+      return st;
     }
-    b->started = 1;
+    b->seen_gen[k] = s->state_gen;
   }
+  b->started = 1;
   int na = 0;
   for (int k = 0; k < n; k++) {
     fg_sim *s = b->sims[k];
@@ -1601,6 +1620,16 @@ extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) 
     fg_batch_dev_result(b->dev, k, &b->cfl[k], &b->force[2 * k]);
     fused_close(b->sims[k], b->L[k], nullptr);
     b->sims[k]->results_ready = 0;  // single-disk stepping re-derives them
+    b->seen_gen[k] = b->sims[k]->state_gen;
+  }
+  for (int k = 0; k < n; k++) {
+    if (!b->active[k]) continue;
+    fg_status st = fg_fused_check(b->sims[k]->ctx, (double)b->cfl[k], &b->force[2 * k]);
+    if (st) {
+      const std::string why = fg_last_error();  // the message buffer is rewritten below
+      host_error("batch disk %d: %s", k, why.c_str());
+      return st;
+    }
   }
   return FG_OK;
 }

@@ -20,6 +20,33 @@
 
 void fg_set_error(const char *fmt, ...);
 
+// Result block of the fused path (fg_ctx::h_fused, coherent pinned host
+// memory the final kernels write straight into): [0..1] star force,
+// FG_FUSED_CFL_SLOT the CFL count (as a double), FG_FUSED_CRASH_SLOT an int
+// that k_fin and the CFL ring pass set to 1 when they meet a negative or
+// non-finite density.
+#define FG_FUSED_SLOTS 8
+#define FG_FUSED_CFL_SLOT 6
+#define FG_FUSED_CRASH_SLOT 7
+
+// Largest CFL sub-cycle count accepted per DT.  The count is
+// ceil(DeltaT / dt) (SourceEuler.cu:228); a count outside 1..FG_CFL_MAX
+// (a time step gone to 0, infinity or NaN) or a non-finite disk force
+// means the state has blown up.
+#define FG_CFL_MAX 10000000
+// FG_OK, or FG_E_CRASH with a message; force2 may be NULL
+static inline fg_status fg_check_substep_results(double cfl, const double *force2) {
+  if (!(cfl >= 1.0 && cfl <= (double)FG_CFL_MAX)) {
+    fg_set_error("crash: CFL sub-cycle count %g outside 1..%d (time step 0, infinite or NaN)", cfl, FG_CFL_MAX);
+    return FG_E_CRASH;
+  }
+  if (force2 && !(__builtin_isfinite(force2[0]) && __builtin_isfinite(force2[1]))) {
+    fg_set_error("crash: non-finite disk force (%g, %g)", force2[0], force2[1]);
+    return FG_E_CRASH;
+  }
+  return FG_OK;
+}
+
 #define FG_HIP(call)                                                                 \
   do {                                                                               \
     hipError_t e_ = (call);                                                          \

@@ -425,15 +425,31 @@ __global__ void k_compute_velocities(double *vr, double *vt, const double *dens,
 // ------------------------------------------------------------------ CFL --
 // ConditionCFLKernel1D + 2D1 + 2D2 (Kernels.cu:1267-1335) for one ring
 // per block: sequential ring mean of vtheta (bit-identical), per-cell
-// time step, ring minimum (min is order independent).
+// time step, ring minimum (min is order independent).  The same pass sets
+// *crash = 1 when a density (or, when energy is not NULL, an energy) of the
+// ring is negative or not finite: the check of DetectCrash/CrashKernel
+// (SourceEuler.cu:453-465, Kernels.cu:953-964, whose call the reference
+// leaves commented out at SourceEuler.cu:266-273), NaN included.
 __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const double *vt, const double *cs,
-                           fg_dev_tables t, int nr, int ns, int fast) {
+                           const double *dens, const double *energy, int *crash, fg_dev_tables t, int nr, int ns,
+                           int fast) {
   extern __shared__ double sh[];
   double *ring = sh;               // ns
   double *red = sh + ns;           // blockDim
   __shared__ double s_vmoy;
   const int i = blockIdx.x;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) ring[j] = vt[(long)i * ns + j];
+  bool bad = false;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    const long l = (long)i * ns + j;
+    ring[j] = vt[l];
+    const double d = dens[l];
+    bad |= !(d >= 0.0 && d < HUGE_VAL);
+    if (energy) {
+      const double e = energy[l];
+      bad |= !(e >= 0.0 && e < HUGE_VAL);
+    }
+  }
+  if (bad) *crash = 1;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = 0.0;
@@ -535,13 +551,21 @@ extern "C" fg_status fg_cfl(fg_ctx *c, double DeltaT, int *ncfl) {
   const int nr = c->nr, ns = c->ns;
   size_t shm = (size_t)(ns + 256) * sizeof(double);
   hipLaunchKernelGGL(k_cfl_ring, dim3(nr), dim3(256), shm, c->stream, c->Vmoy, c->newDT, F(c, FG_VRAD),
-                     F(c, FG_VTHETA), F(c, FG_SOUNDSPEED), c->t, nr, ns, c->p.fast_transport);
+                     F(c, FG_VTHETA), F(c, FG_SOUNDSPEED), F(c, FG_DENS),
+                     c->p.adiabatic ? (const double *)F(c, FG_ENERGY) : (const double *)nullptr, c->cfl_out + 1, c->t,
+                     nr, ns, c->p.fast_transport);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, c->cfl_out, c->DT1D, c->Vmoy, c->newDT,
                      c->t.invRmed, c->t.dt1d_coef, DeltaT, nr);
   FG_LAUNCH_CHECK();
-  FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FG_HIP(hipMemcpyAsync(c->h_cfl, c->cfl_out, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   FG_HIP(hipStreamSynchronize(c->stream));
+  if (c->h_cfl[1]) {  // reported once; the next pass over a bad state flags it again
+    FG_HIP(hipMemsetAsync(c->cfl_out + 1, 0, sizeof(int), c->stream));
+    fg_set_error("crash: negative or non-finite density or energy");
+    return FG_E_CRASH;
+  }
+  if (fg_status s = fg_check_substep_results((double)c->h_cfl[0], nullptr)) return s;
   *ncfl = c->h_cfl[0];
   return FG_OK;
 }

@@ -13,21 +13,54 @@
  * OpenMP pragmas parallelise only per-cell loops (no cross-cell
  * accumulation), so the -fopenmp build gives bit-identical results.
  *
- * Parity pins (what checks each part, tests/test_oracle_*.py):
- *   parser            known-answer on the reference decks in/*.par, *.cfg
+ * Parity pins: every routine below has a check the mathematics fixes,
+ * chosen so that a dropped term, a wrong sign or factor or a shifted index
+ * fails (tests/test_oracle_pins.py, test_oracle_physics_pins.py,
+ * test_oracle_diagnostics.py):
+ *   parser            known answers on the reference decks in/*.par, *.cfg
  *   grid              closed forms: sum Surf*Ns = pi(Rmax^2-Rmin^2), Rmed
- *   initial state     closed-form point values of Sigma, vtheta, vrad
- *   DeviceReduce      exact (Fraction) sums; order replayed independently
- *   CFL               brute-force min over cells; Keplerian closed form
- *   Substep1          uniform-Sigma zero-potential known answer
- *   viscous stress    solid-body rotation gives zero shear
- *   artificial visc.  expanding flow gives exactly zero
- *   transport         mass conservation to round-off, integer-Courant
- *                     azimuthal advection equals an exact shift, TVD
- *   evanescent        closed-form damping update, untouched rings bitwise
- *   RK5 (Cash-Karp)   circular Kepler orbit: radius, energy, period
+ *   initial state     closed-form Sigma (with cavity), vtheta, vrad, energy
+ *   EOS               c_s, P, T closed forms (isothermal and adiabatic)
+ *   aspect ratio      transition-zone closed form (through the smoothing)
+ *   DeviceReduce      exact (Fraction) sums; order replayed by hand
+ *   CFL               brute-force min over cells
+ *   potential         softened point mass + both indirect terms at every
+ *                     cell, planet on a cell (d = 0), star alone, Roche
+ *   Substep1          gradients of linear P and Phi (radial, azimuthal),
+ *                     centrifugal term in a rotating frame, imposed drift
+ *   viscous stress    power-law shear flows (tau_rphi = rho nu r dOmega/dr),
+ *                     radial compression v_r ~ r^2 (tau_rr, div v)
+ *   viscous update    their divergences: nu A (q^2-1) r^(q-2), 4 nu v0
+ *   alpha viscosity   nu = alpha c_s^2 / Omega_K, ring mean, cavity taper
+ *   artificial visc.  uniform compression: CVNR^2 rho dv^2, -grad q / rho,
+ *                     the compressional heating; expansion gives zero
+ *   Substep3          backward-Euler compression and cooling relaxation,
+ *                     equilibrium fixed point, shear-flow heating, ring-0
+ *                     power-law extrapolation
+ *   radial transport  brute-force van Leer column, divergence-free flow
+ *                     keeps uniform Sigma, mass conservation
+ *   azimuthal transp. brute-force FARGO ring (mean shift, uniform and
+ *                     residual sweeps), integer-Courant shift, TVD
+ *   ComputeVelocities identity for a vanishing step (rotating frame)
+ *   boundaries        evanescent closed form; open edge ghost values;
+ *                     non-reflecting shift, mean reset and characteristic
+ *                     face velocity; sub-Keplerian equilibrium rotation
+ *   frames            corotating (planet fixed), guiding centre (frame at
+ *                     the mean motion, mean longitude fixed); CorrectVtheta
+ *   planets (RK5)     circular Kepler orbit; forced circular rotation;
+ *                     release migration; planet-planet term (quirk)
+ *   AlgoGas           MassTaper 0 on the first substep, substeps add up to
+ *                     DT, NTOT+1 calls
  *   FFT               numpy.fft on random data; brute-force convolution
- * Parity unpinned (no reference output exists; the reference needs nvcc):
+ *   self-gravity      the finite-difference gradient of the direct sum of
+ *                     softened log-polar cell potentials; the velocity kick
+ *                     (linear interpolation) and the ring mean
+ *   disk force, Hill  point-mass closed forms, split radius, circumplanetary
+ *                     mass (test_oracle_diagnostics.py)
+ * Golden fixtures (tests/golden/) are written by this oracle: they guard
+ * against regressions only.
+ * Reference behaviour no check here can reach (the reference needs nvcc
+ * and an NVIDIA GPU, and ships no outputs), decided as documented:
  *   ulp behaviour of CUDA's device libm inside kernels; the layout-
  *   dependent ViscosityAlpha pun (read here as "alpha viscosity on" in
  *   FViscosity, "off" in the initial-velocity kernel, as the reference's
