@@ -73,6 +73,10 @@ extern "C" fg_status fg_create(const fg_params *p, const fg_tables *t, int devic
     return FG_E_INVALID;
   }
   FG_HIP(hipSetDevice(device));
+  // this device's copy of the allocation-free entry point's scratch array;
+  // a failure is reported by that entry point (fg_disk_force_hill_noalloc),
+  // the context itself does not need it
+  (void)fg_noalloc_setup(device);
   fg_ctx *c = new fg_ctx();
   c->tile_src_th = 16;
   c->tile_trad_band = 16;

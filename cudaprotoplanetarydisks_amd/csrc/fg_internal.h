@@ -19,6 +19,9 @@
 #define FG_HILL_MAX 64
 
 void fg_set_error(const char *fmt, ...);
+// once per device, with that device current: locate the device's copy of
+// the scratch array of fg_disk_force_hill_noalloc (fg_reduce.hip)
+fg_status fg_noalloc_setup(int device);
 
 // Result block of the fused path (fg_ctx::h_fused, coherent pinned host
 // memory the final kernels write straight into): [0..1] star force,

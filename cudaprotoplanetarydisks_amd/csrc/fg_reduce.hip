@@ -1,6 +1,9 @@
 // Single-context reductions: DeviceReduce-order sums and the disk force
 // (the device functions live in fg_reduce.h), and the non-reflecting
 // boundary, whose ring means are such sums.
+#include <atomic>
+#include <mutex>
+
 #include "fg_reduce.h"
 
 template <class Src, int CH>
@@ -105,6 +108,75 @@ extern "C" fg_status fg_disk_force_hill(fg_ctx *c, double x, double y, double rs
   FG_LAUNCH_CHECK();
   FG_HIP(hipMemcpyAsync(globalforce, c->hill_out, sizeof(double) * 4 * dimfxy, hipMemcpyDeviceToHost, c->stream));
   FG_HIP(hipStreamSynchronize(c->stream));
+  return FG_OK;
+}
+
+// ---- fg_disk_force_hill without allocation ----------------------------
+// The block partials of fg_disk_force_hill_noalloc live in a file-scope
+// device array: 4 * FG_HILL_MAX rows of FG_RED_BLOCKS block sums (2 MiB),
+// enough for every grid (DeviceReduce never uses more than FG_RED_BLOCKS
+// blocks) and every dimfxy <= FG_HILL_MAX.  Each GPU holds its own copy
+// until the process ends.  On the host the array's name is not a device
+// address: fg_noalloc_setup (called by fg_create with the context's device
+// current) asks hipGetSymbolAddress for this device's copy once and keeps
+// the pointer; kernels only ever receive that pointer.  Nothing in it needs
+// clearing: k_hill_partial writes every row and block k_hill_final reads.
+// All calls on one GPU share the array, so calls that overlap on one GPU
+// (two streams, two threads) race.
+#define FG_NOALLOC_DOUBLES (4 * FG_HILL_MAX * FG_RED_BLOCKS)
+#define FG_NOALLOC_DEVICES 64
+static __device__ __align__(256) double scratch[FG_NOALLOC_DOUBLES];
+static std::atomic<double *> g_scratch_dev[FG_NOALLOC_DEVICES];
+static std::mutex g_scratch_mu;
+
+fg_status fg_noalloc_setup(int device) {
+  if (device < 0 || device >= FG_NOALLOC_DEVICES) {
+    fg_set_error("device %d beyond the %d the allocation-free entry point supports", device, FG_NOALLOC_DEVICES);
+    return FG_E_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device].load()) return FG_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fg_set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    return FG_E_HIP;
+  }
+  g_scratch_dev[device].store((double *)p);
+  return FG_OK;
+}
+
+extern "C" fg_status fg_disk_force_hill_noalloc(fg_ctx *c, double x, double y, double rsmoothing, int dimfxy,
+                                                double a, double rh, double *globalforce_dev) {
+  if (!c || !globalforce_dev) {
+    fg_set_error("fg_disk_force_hill_noalloc: NULL argument");
+    return FG_E_INVALID;
+  }
+  if (dimfxy < 1 || dimfxy > FG_HILL_MAX) {
+    fg_set_error("fg_disk_force_hill_noalloc: dimfxy = %d outside 1..%d (the scratch array holds %d variants)",
+                 dimfxy, FG_HILL_MAX, FG_HILL_MAX);
+    return FG_E_INVALID;
+  }
+  double *scr = (c->device >= 0 && c->device < FG_NOALLOC_DEVICES) ? g_scratch_dev[c->device].load() : nullptr;
+  if (!scr) {
+    fg_set_error("fg_disk_force_hill_noalloc: fg_create could not locate the scratch array on device %d",
+                 c->device);
+    return FG_E_HIP;
+  }
+  const long n = (long)c->nr * c->ns;
+  const int B = red_blocks(n);
+  if ((long)4 * dimfxy * B > FG_NOALLOC_DOUBLES) {
+    fg_set_error("fg_disk_force_hill_noalloc: %d x %d partial sums exceed the %d-double scratch array", 4 * dimfxy,
+                 B, FG_NOALLOC_DOUBLES);
+    return FG_E_INVALID;
+  }
+  hill_force_src src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, x, y, rsmoothing, a, rh,
+                     c->ns, dimfxy};
+  hipLaunchKernelGGL(k_hill_partial, dim3(B, dimfxy), dim3(RED_TPB), 0, c->stream, src, n, B, scr);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_hill_final, dim3((4 * dimfxy + 63) / 64), dim3(64), 0, c->stream, scr, B, dimfxy,
+                     globalforce_dev);
+  FG_LAUNCH_CHECK();
   return FG_OK;
 }
 

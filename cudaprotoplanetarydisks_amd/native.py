@@ -72,6 +72,7 @@ SIGNATURES = {
     "fg_disk_force_hill": (ctypes.c_int, [_vp, _d, _d, _d, ctypes.c_int, _d, _d, _pd]),
     "fg_check_shared_division": (ctypes.c_int, [_pd, _pd, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "fg_circumplanetary_mass": (ctypes.c_int, [_vp, _d, _d, _d, _pd]),
+    "fg_disk_force_hill_noalloc": (ctypes.c_int, [_vp, _d, _d, _d, ctypes.c_int, _d, _d, _vp]),
     "fg_potential": (ctypes.c_int, [_vp, ctypes.POINTER(FgPlanetPot), ctypes.c_int, _d, _d]),
     "fg_correct_vtheta": (ctypes.c_int, [_vp, _d]),
     "fg_boundary": (ctypes.c_int, [_vp, _d, _d]),

@@ -177,6 +177,17 @@ fg_status fg_disk_force(fg_ctx *ctx, double x, double y, double rsmoothing, doub
  * 1 <= dimfxy <= 64.  Used by UpdateLog (Force.cu:19). */
 fg_status fg_disk_force_hill(fg_ctx *ctx, double x, double y, double rsmoothing, int dimfxy, double a, double rh,
                              double *globalforce);
+/* No counterpart in the reference: fg_disk_force_hill for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing:
+ * the block partial sums go to a 2 MiB scratch array built into the
+ * library (one per GPU, kept until the process ends), and the
+ * 4 * dimfxy results, same layout and bits as fg_disk_force_hill, go to
+ * globalforce_dev, a device pointer the caller owns.  Asynchronous on the
+ * context's stream (fg_sync waits).  dimfxy <= FG_HILL_MAX (64); every grid
+ * fits.  Calls that overlap on one GPU share the scratch array and race. */
+fg_status fg_disk_force_hill_noalloc(fg_ctx *ctx, double x, double y, double rsmoothing, int dimfxy, double a,
+                                     double rh, double *globalforce_dev);
 /* CircumPlanetaryMass, SourceEuler.cu:549-563 (Kernels.cu:236-251): disk
  * mass within hill_radius of (xpl, ypl), in DeviceReduce order */
 fg_status fg_circumplanetary_mass(fg_ctx *ctx, double xpl, double ypl, double hill_radius, double *mass);
