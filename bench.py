@@ -51,12 +51,23 @@ def write_workload_deck(tmp, nr, ns):
     return write_deck(os.path.join(tmp, "workload.par"), Nrad=nr, Nsec=ns, OutputDir=tmp + "/")
 
 
-def cpu_baseline(deck, nr, ns, steps, warm):
-    """Oracle (FP64, OpenMP per-cell loops, same arithmetic) on the same
-    noisy initial state."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(deck, nr, ns, steps, warm, omp=True):
+    """Oracle (FP64, same arithmetic; OpenMP per-cell loops, or the serial
+    build) on the same noisy initial state."""
     import numpy as np
     from oracle_binding import Oracle
-    orc = Oracle(deck, omp=True)
+    orc = Oracle(deck, omp=omp)
     dens = orc.view("Dens")
     sig = orc.field("SigmaMed")
     idx = (np.arange(nr, dtype=np.uint64)[:, None] * np.uint64(ns) + np.arange(ns, dtype=np.uint64)[None, :])
@@ -140,6 +151,40 @@ def gpu_leg(deck, precision, path, steps, warmup):
         sim.close()
 
 
+def adsg_leg(steps, warmup):
+    """The ADSG workload (decks/c2_adsg.par: 512 x 1536, adiabatic with beta
+    cooling, alpha viscosity, self-gravity on the 1024 x 1536 padded grid,
+    a planet feeling the disk) on the per-routine FP64 path: zone-updates/s
+    and the share of HBM peak at the compulsory 64 B/zone (Sigma, v_r,
+    v_theta, e read and written) + 576 B/zone of self-gravity FFT traffic."""
+    from cudaprotoplanetarydisks_amd import Simulation
+    from cudaprotoplanetarydisks_amd.native import call
+    from cudaprotoplanetarydisks_amd.workloads import DECKS
+    with tempfile.TemporaryDirectory() as tmp:
+        src = open(os.path.join(DECKS, "c2_adsg.par")).read()
+        src = src.replace("jupiter_feeldisk.cfg", os.path.join(DECKS, "jupiter_feeldisk.cfg"))
+        src = src.replace("out_c2/", tmp + "/")
+        deck = os.path.join(tmp, "c2_adsg.par")
+        with open(deck, "w") as f:
+            f.write(src)
+        with Simulation(deck) as sim:
+            ctx = sim.ctx.h
+            for _ in range(warmup):
+                sim.substep()
+            call("fg_sync", ctx)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.substep()
+            call("fg_sync", ctx)
+            el = time.perf_counter() - t0
+            zones = sim.nrad * sim.nsec
+        zus = zones * steps / el
+        return {"workload": "ADSG + self-gravity, 512x1536 (decks/c2_adsg.par), per-routine FP64 path",
+                "steps": steps, "ms_per_step": round(1e3 * el / steps, 4), "value": round(zus, 1),
+                "unit": "zone-updates/s", "b_alg_per_zone": 64 + 576,
+                "pct_hbm_peak": round(100.0 * zus * (64 + 576) / (HBM_PEAK_GBS * 1e9), 3)}
+
+
 def roofline_of(prof, nr, ns, precision):
     """Roofline entry of the dominant (longest) kernel: algorithmic bytes
     per launch over its event-timed duration.  Both precisions keep the
@@ -169,10 +214,12 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--path", choices=["fused", "routine"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=160)
+    ap.add_argument("--cpu-serial-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-alt", action="store_true", help="skip the leg in the other precision")
     ap.add_argument("--ensemble-disks", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--adsg-steps", type=int, default=10, help="substeps of the ADSG leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -194,14 +241,20 @@ def main():
         ens = None
         if args.ensemble_disks > 0:
             ens = ensemble_leg(args.ensemble_disks, args.ensemble_steps, 10)
+        adsg = adsg_leg(args.adsg_steps, 2) if args.adsg_steps > 0 else None
         cpu = None
         if not args.no_cpu:
             threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
             os.environ["OMP_NUM_THREADS"] = str(threads)
-            rate, secs = cpu_baseline(deck, nr, ns, args.cpu_steps, 1)
+            rate, secs = cpu_baseline(deck, nr, ns, args.cpu_steps, 2)
             cpu = {"value": round(rate, 1), "unit": "zone-updates/s", "cores": threads, "kind": "port",
-                   "sample": f"oracle FP64 (OpenMP), {nr}x{ns}, 1 warm-up + {args.cpu_steps} substeps "
-                             f"({secs:.1f} s)"}
+                   "sample": f"oracle FP64 (OpenMP), {nr}x{ns}, 2 warm-up + {args.cpu_steps} substeps "
+                             f"({secs:.1f} s)", "cpu_model": cpu_model()}
+            if args.cpu_serial_steps > 0:
+                srate, ssecs = cpu_baseline(deck, nr, ns, args.cpu_serial_steps, 2, omp=False)
+                cpu["serial"] = {"value": round(srate, 1), "unit": "zone-updates/s", "cores": 1,
+                                 "sample": f"serial oracle, {nr}x{ns}, 2 warm-up + {args.cpu_serial_steps} "
+                                           f"substeps ({ssecs:.1f} s)"}
     b_alg = 48  # read + write of Sigma, vrad, vtheta (FP64) per zone-update
     out = {
         "metric": METRIC, "value": round(zus, 1), "unit": "zone-updates/s", "n_gpus": 1,
@@ -213,7 +266,7 @@ def main():
                    "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision,
                    "state": "fp64"},
         "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
-        "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens,
+        "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "adsg": adsg,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
         "finite": finite,
         "other_precision": alt,
