@@ -151,6 +151,44 @@ def gpu_leg(deck, precision, path, steps, warmup):
         sim.close()
 
 
+def deck_copy(tmp, name):
+    """A committed deck with its planet file and output directory made
+    absolute (the decks name them relative to the decks directory)."""
+    from cudaprotoplanetarydisks_amd.workloads import DECKS
+    src = open(os.path.join(DECKS, name)).read()
+    for cfg in ("jupiter_feeldisk.cfg", "jupiter_fixed.cfg"):
+        src = src.replace(cfg, os.path.join(DECKS, cfg))
+    src = src.replace("out_c1/", tmp + "/").replace("out_c2/", tmp + "/")
+    deck = os.path.join(tmp, name)
+    with open(deck, "w") as f:
+        f.write(src)
+    return deck
+
+
+def small_grid_leg(steps, warmup):
+    """The plain-FARGO workload at its own 128 x 384 grid (decks/c1_plain_fargo.par)
+    on the fused FP64 path: at this size the per-substep host round trip
+    (CFL count and star force back to the host, planets advanced there,
+    the next substep launched) is part of the time."""
+    from cudaprotoplanetarydisks_amd import Simulation
+    from cudaprotoplanetarydisks_amd.native import call
+    with tempfile.TemporaryDirectory() as tmp:
+        with Simulation(deck_copy(tmp, "c1_plain_fargo.par"), fused=True) as sim:
+            ctx = sim.ctx.h
+            for _ in range(warmup):
+                sim.substep()
+            call("fg_sync", ctx)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.substep()
+            call("fg_sync", ctx)
+            el = time.perf_counter() - t0
+            zones = sim.nrad * sim.nsec
+        return {"workload": "plain FARGO, 128x384 (decks/c1_plain_fargo.par), fused FP64 path",
+                "steps": steps, "ms_per_step": round(1e3 * el / steps, 4),
+                "value": round(zones * steps / el, 1), "unit": "zone-updates/s"}
+
+
 def adsg_leg(steps, warmup):
     """The ADSG workload (decks/c2_adsg.par: 512 x 1536, adiabatic with beta
     cooling, alpha viscosity, self-gravity on the 1024 x 1536 padded grid,
@@ -159,15 +197,8 @@ def adsg_leg(steps, warmup):
     v_theta, e read and written) + 576 B/zone of self-gravity FFT traffic."""
     from cudaprotoplanetarydisks_amd import Simulation
     from cudaprotoplanetarydisks_amd.native import call
-    from cudaprotoplanetarydisks_amd.workloads import DECKS
     with tempfile.TemporaryDirectory() as tmp:
-        src = open(os.path.join(DECKS, "c2_adsg.par")).read()
-        src = src.replace("jupiter_feeldisk.cfg", os.path.join(DECKS, "jupiter_feeldisk.cfg"))
-        src = src.replace("out_c2/", tmp + "/")
-        deck = os.path.join(tmp, "c2_adsg.par")
-        with open(deck, "w") as f:
-            f.write(src)
-        with Simulation(deck) as sim:
+        with Simulation(deck_copy(tmp, "c2_adsg.par")) as sim:
             ctx = sim.ctx.h
             for _ in range(warmup):
                 sim.substep()
@@ -220,6 +251,7 @@ def main():
     ap.add_argument("--ensemble-disks", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--adsg-steps", type=int, default=10, help="substeps of the ADSG leg (0: skip)")
+    ap.add_argument("--small-steps", type=int, default=1000, help="substeps of the 128x384 leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -242,6 +274,7 @@ def main():
         if args.ensemble_disks > 0:
             ens = ensemble_leg(args.ensemble_disks, args.ensemble_steps, 10)
         adsg = adsg_leg(args.adsg_steps, 2) if args.adsg_steps > 0 else None
+        small = small_grid_leg(args.small_steps, 20) if args.small_steps > 0 else None
         cpu = None
         if not args.no_cpu:
             threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -266,7 +299,7 @@ def main():
                    "nrad": nr, "nsec": ns, "path": args.path, "precision": args.precision,
                    "state": "fp64"},
         "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
-        "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "adsg": adsg,
+        "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "adsg": adsg, "small_grid": small,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
         "finite": finite,
         "other_precision": alt,

@@ -30,12 +30,14 @@ extern "C" void fg_fft_twiddles(int L, int sign, double *cs);  // fg_hostmath.cp
 struct fg_fft_plan {
   int n, nst;
   int p[FG_FFT_MAXST], L[FG_FFT_MAXST], S[FG_FFT_MAXST], off[FG_FFT_MAXST];
+  int sh[FG_FFT_MAXST];  // log2 S[d] when every S[d] is a power of two
+  int pow2s;             // every level above the deepest is radix 2 (k_fft_pow2 applies)
   double2 *tw[2];  // per-level twiddle tables; [0] forward (sign -1), [1] inverse (+1)
 };
 
 struct fft_args {
   int nst;
-  int p[FG_FFT_MAXST], L[FG_FFT_MAXST], S[FG_FFT_MAXST], off[FG_FFT_MAXST];
+  int p[FG_FFT_MAXST], L[FG_FFT_MAXST], S[FG_FFT_MAXST], off[FG_FFT_MAXST], sh[FG_FFT_MAXST];
   const double2 *tw;
   double2 *data[2];     // up to two grids transformed by one launch (blockIdx.y)
   int n;                // transform length
@@ -92,6 +94,127 @@ __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
   for (int e = tid; e < E; e += FFT_TPB) {
     const int cb = e % CB, o = e / CB, line = line0 + cb;
     if (line < a.nlines) data[line * a.line_step + o * a.elem_stride] = buf[e];
+  }
+}
+
+// k_fft_pow2 runs 1024 threads with 4 outputs each (FFT2_TPB * FFT2_U =
+// FFT_MAX_ELEMS): four complex outputs and their operands fit in few
+// registers, so several workgroups share a CU.
+#define FFT2_TPB 1024
+#define FFT2_U 4
+
+// The same transform for lengths N = 2^a q with at most one factor other
+// than 2 (q = 1, 3, 5 or a prime done as one direct DFT): every level above
+// the deepest is radix 2, so each S_d is a power of two and an output's
+// indices are shifts and masks; k = kk mod m_d is kk or kk - m_d for the
+// radix-2 levels and 0 on the deepest level (m = 1).  CB lines are staged
+// together (element o of line cb at buf[o*CB + cb]); global loads and
+// stores run along the line when the line is contiguous (rows) and across
+// the CB lines otherwise (columns).  Outputs are accumulated exactly as in
+// k_fft_lines (same twiddles, same order), so both give the same bits.
+template <int CB>
+__global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
+  extern __shared__ double2 buf[];
+  constexpr int LCB = CB == 1 ? 0 : CB == 2 ? 1 : CB == 4 ? 2 : 3;
+  const int N = a.n, tid = threadIdx.x;
+  const int line0 = blockIdx.x * CB;
+  double2 *data = a.data[blockIdx.y];
+  const int E = N * CB;
+  if (a.elem_stride == 1) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const int line = line0 + cb;
+      const double2 *src = data + (long)line * a.line_step;
+      for (int o = tid; o < N; o += FFT2_TPB)
+        buf[o * CB + cb] = (line < a.nlines) ? src[o] : make_double2(0.0, 0.0);
+    }
+  } else {
+    for (int e = tid; e < E; e += FFT2_TPB) {
+      const int cb = e & (CB - 1), o = e >> LCB, line = line0 + cb;
+      buf[e] = (line < a.nlines) ? data[line * a.line_step + o * a.elem_stride] : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  double2 out[FFT2_U];
+  {
+    // deepest level: L = p (any factor), S = N / p, m = 1, k = 0
+    const int d = a.nst - 1;
+    const int p = a.p[d], L = a.L[d], sh = a.sh[d], S = 1 << sh;
+    const double2 *w = a.tw + a.off[d];
+#pragma unroll
+    for (int u = 0; u < FFT2_U; u++) {
+      const int e = tid + u * FFT2_TPB;
+      out[u] = make_double2(0.0, 0.0);
+      if (e < E) {
+        const int cb = e & (CB - 1), o = e >> LCB;
+        const int s = o & (S - 1), kk = o >> sh;
+        double re = 0.0, im = 0.0;
+        int idx = 0;  // (r*kk) mod L
+        for (int r = 0; r < p; r++) {
+          const double2 x = buf[(s + S * r) * CB + cb];
+          const double2 t = w[idx];
+          re += x.x * t.x - x.y * t.y;
+          im += x.x * t.y + x.y * t.x;
+          idx += kk;
+          if (idx >= L) idx -= L;
+        }
+        out[u] = make_double2(re, im);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < FFT2_U; u++) {
+      const int e = tid + u * FFT2_TPB;
+      if (e < E) buf[e] = out[u];
+    }
+    __syncthreads();
+  }
+  // the radix-2 levels above it: r = 0 with w[0], r = 1 with w[kk] (kk < L)
+  for (int d = a.nst - 2; d >= 0; d--) {
+    const int L = a.L[d], sh = a.sh[d], m = L >> 1;
+    const int S = 1 << sh;
+    const double2 *w = a.tw + a.off[d];
+    const double2 w0 = w[0];
+#pragma unroll
+    for (int u = 0; u < FFT2_U; u++) {
+      const int e = tid + u * FFT2_TPB;
+      if (e < E) {
+        const int cb = e & (CB - 1), o = e >> LCB;
+        const int s = o & (S - 1), kk = o >> sh;
+        const int k = kk >= m ? kk - m : kk;
+        const int base = s + 2 * S * k;
+        const double2 x0 = buf[base * CB + cb];
+        const double2 x1 = buf[(base + S) * CB + cb];
+        const double2 t1 = w[kk];
+        double re = 0.0, im = 0.0;
+        re += x0.x * w0.x - x0.y * w0.y;
+        im += x0.x * w0.y + x0.y * w0.x;
+        re += x1.x * t1.x - x1.y * t1.y;
+        im += x1.x * t1.y + x1.y * t1.x;
+        out[u] = make_double2(re, im);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < FFT2_U; u++) {
+      const int e = tid + u * FFT2_TPB;
+      if (e < E) buf[e] = out[u];
+    }
+    __syncthreads();
+  }
+  if (a.elem_stride == 1) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const int line = line0 + cb;
+      if (line >= a.nlines) continue;
+      double2 *dst = data + (long)line * a.line_step;
+      for (int o = tid; o < N; o += FFT2_TPB) dst[o] = buf[o * CB + cb];
+    }
+  } else {
+    for (int e = tid; e < E; e += FFT2_TPB) {
+      const int cb = e & (CB - 1), o = e >> LCB, line = line0 + cb;
+      if (line < a.nlines) data[line * a.line_step + o * a.elem_stride] = buf[e];
+    }
   }
 }
 
@@ -188,11 +311,16 @@ static fg_status plan_create(int n, fg_fft_plan **out) {
     rem /= p;
   }
   pl->nst = k;
+  pl->pow2s = k >= 1;
+  for (int d = 0; d + 1 < k; d++)
+    if (pl->p[d] != 2) pl->pow2s = 0;
   int L = 1, total = 0;
   for (int d = k - 1; d >= 0; d--) {
     L *= pl->p[d];
     pl->L[d] = L;
     pl->S[d] = n / L;
+    pl->sh[d] = 0;
+    while ((1 << pl->sh[d]) < pl->S[d]) pl->sh[d]++;
     pl->off[d] = total;
     total += L;
   }
@@ -237,26 +365,39 @@ static fg_status fft2d(fg_ctx *c, double *g0, double *g1, int inverse) {
       a.L[d] = pl->L[d];
       a.S[d] = pl->S[d];
       a.off[d] = pl->off[d];
+      a.sh[d] = pl->sh[d];
     }
     a.tw = pl->tw[inverse ? 1 : 0];
     a.data[0] = (double2 *)g0;
     a.data[1] = (double2 *)(g1 ? g1 : g0);
     a.n = pl->n;
-    if (pass == 0) {  // rows: contiguous, one per workgroup
+    // lines staged per workgroup: up to FFT_MAX_ELEMS elements, a power of
+    // two up to 8 for k_fft_pow2 (rows 2 at a time, columns 4 at NSEC 1536)
+    int cb = FFT_MAX_ELEMS / pl->n;
+    cb = cb >= 8 ? 8 : cb >= 4 ? 4 : cb >= 2 ? 2 : 1;
+    if (pass == 0) {  // rows: contiguous
       a.nlines = n1;
       a.elem_stride = 1;
       a.line_step = n2;
-      a.lines_per_block = 1;
+      a.lines_per_block = pl->pow2s ? (cb > 2 ? 2 : cb) : 1;
     } else {  // columns: stride NSEC, adjacent columns staged together
       a.nlines = n2;
       a.elem_stride = n2;
       a.line_step = 1;
-      int cb = FFT_MAX_ELEMS / n1;
-      a.lines_per_block = cb > 8 ? 8 : cb;
+      a.lines_per_block = cb;
     }
     const int blocks = (a.nlines + a.lines_per_block - 1) / a.lines_per_block;
     const size_t shm = (size_t)a.n * a.lines_per_block * sizeof(double2);
-    hipLaunchKernelGGL(k_fft_lines, dim3(blocks, ngrids), dim3(FFT_TPB), shm, c->stream, a);
+    if (!pl->pow2s)
+      hipLaunchKernelGGL(k_fft_lines, dim3(blocks, ngrids), dim3(FFT_TPB), shm, c->stream, a);
+    else if (a.lines_per_block == 8)
+      hipLaunchKernelGGL(k_fft_pow2<8>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
+    else if (a.lines_per_block == 4)
+      hipLaunchKernelGGL(k_fft_pow2<4>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
+    else if (a.lines_per_block == 2)
+      hipLaunchKernelGGL(k_fft_pow2<2>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_fft_pow2<1>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
     FG_LAUNCH_CHECK();
   }
   return FG_OK;
