@@ -216,6 +216,21 @@ def adsg_leg(steps, warmup):
                 "pct_hbm_peak": round(100.0 * zus * (64 + 576) / (HBM_PEAK_GBS * 1e9), 3)}
 
 
+def emulator_leg(n_eval, max_orbits):
+    """Ensemble time-to-steady, emulator warm start against plain cold start:
+    held-out draws (128 x 384 disks, the training seed + 1) evolved by the
+    batched solver from both starts with the committed weights
+    (cudaprotoplanetarydisks_amd/data/emulator_128x384.npz, trained on an
+    MI355X by the project's BF16-MFMA kernels on 512 batched runs of 150
+    orbits).  Steady: the mass-weighted L1 change of the orbit-averaged ring
+    profile per orbit below eps."""
+    from cudaprotoplanetarydisks_amd.emulator import WEIGHTS, warm_start_leg
+    t0 = time.perf_counter()
+    rep = warm_start_leg(WEIGHTS, n_eval, max_orbits, [1e-3, 5e-4, 2e-4])
+    rep["seconds"] = round(time.perf_counter() - t0, 1)
+    return rep
+
+
 def roofline_of(prof, nr, ns, precision):
     """Roofline entry of the dominant (longest) kernel: algorithmic bytes
     per launch over its event-timed duration.  Both precisions keep the
@@ -252,6 +267,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--adsg-steps", type=int, default=10, help="substeps of the ADSG leg (0: skip)")
     ap.add_argument("--small-steps", type=int, default=1000, help="substeps of the 128x384 leg (0: skip)")
+    ap.add_argument("--emulator-disks", type=int, default=16, help="held-out disks of the warm-start leg (0: skip)")
+    ap.add_argument("--emulator-orbits", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -275,6 +292,7 @@ def main():
             ens = ensemble_leg(args.ensemble_disks, args.ensemble_steps, 10)
         adsg = adsg_leg(args.adsg_steps, 2) if args.adsg_steps > 0 else None
         small = small_grid_leg(args.small_steps, 20) if args.small_steps > 0 else None
+        emu = emulator_leg(args.emulator_disks, args.emulator_orbits) if args.emulator_disks > 0 else None
         cpu = None
         if not args.no_cpu:
             threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -300,6 +318,7 @@ def main():
                    "state": "fp64"},
         "pct_hbm_peak": round(100.0 * zus * b_alg / (HBM_PEAK_GBS * 1e9), 3),
         "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "adsg": adsg, "small_grid": small,
+        "emulator": emu,
         "gpu_vs_cpu": round(zus / cpu["value"], 1) if cpu else None,
         "finite": finite,
         "other_precision": alt,

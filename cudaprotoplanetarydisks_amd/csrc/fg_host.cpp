@@ -299,6 +299,9 @@ struct fg_sim {
   // restart, a single-disk substep); a batch re-derives the CFL count and
   // star force of a disk whose generation it has not seen
   unsigned long state_gen = 0;
+  // a batch found this disk's state blown up (FG_E_CRASH) and stopped
+  // advancing it; a new state (fg_sim_set_state, fg_sim_restart) clears it
+  int crashed = 0;
 };
 
 namespace {
@@ -1379,6 +1382,7 @@ extern "C" fg_status fg_sim_restart(fg_sim *s, int idx) {
   s->in_dt = 0;
   s->results_ready = 0;
   s->state_gen++;
+  s->crashed = 0;
   s->planets_fresh = 1;  // keep the planet files: their lines up to idx are this run's history
   CHECK(fg_upload_state(s->ctx, dens.data(), vr.data(), vt.data(), d.Adiabatic ? en.data() : nullptr));
   CHECK(fg_sound_speed(s->ctx));
@@ -1426,6 +1430,7 @@ extern "C" fg_status fg_sim_set_state(fg_sim *s, const double *dens, const doubl
   if (!s) return FG_E_INVALID;
   s->results_ready = 0;
   s->state_gen++;
+  s->crashed = 0;
   return fg_upload_state(s->ctx, dens, vrad, vtheta, energy);
 }
 
@@ -1444,6 +1449,7 @@ extern "C" fg_status fg_sim_get(fg_sim *s, const char *name, double *v) {
              {"IndirectTerm.y", s->IndirectY},  {"HillRadius", s->HillRadius},
              {"Adiabatic", (double)s->d.Adiabatic}, {"substeps", (double)s->nsub}, {"SelfGravity", (double)s->d.SelfGravity},
              {"fused_active", (double)s->fused_active}, {"mdcp", s->mdcp}, {"mdcp1", s->mdcp1},
+             {"crashed", (double)s->crashed},
              {"exces_mdcp", s->exces_mdcp}};
   for (auto &e : tab)
     if (strcmp(e.k, name) == 0) {
@@ -1574,19 +1580,43 @@ extern "C" fg_status fg_batch_set_tiles(fg_batch *b, int k, int src_rows, int tr
   return FG_OK;
 }
 
+// A disk whose state blows up (FG_E_CRASH: a negative or non-finite
+// density, or a CFL count or star force showing a time step gone to 0,
+// infinity or NaN) is marked crashed and left alone; the other disks go
+// on.  The batch substep in which that happens returns FG_E_CRASH naming
+// the disks; fg_sim_get(disk, "crashed") tells them apart afterwards.
+static void batch_crash(fg_batch *b, int k, std::string &msg, int &ncrash) {
+  b->sims[k]->crashed = 1;
+  b->active[k] = 0;
+  if (ncrash++ < 8) {
+    if (!msg.empty()) msg += "; ";
+    msg += "batch disk " + std::to_string(k) + ": " + fg_last_error();
+  }
+}
+
+static fg_status batch_crash_status(const std::string &msg, int ncrash) {
+  if (!ncrash) return FG_OK;
+  host_error("%s%s", msg.c_str(), ncrash > 8 ? " (and more)" : "");
+  return FG_E_CRASH;
+}
+
 extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) {
   if (!b) return FG_E_INVALID;
   const int n = (int)b->sims.size();
+  std::string crash_msg;
+  int ncrash = 0;
   // CFL count and star force of every initial state, and again for a disk
   // whose state changed since its last batch step (fg_sim_set_state,
   // fg_sim_restart or single-disk substeps through fg_batch_sim)
   for (int k = 0; k < n; k++) {
     fg_sim *s = b->sims[k];
-    if (b->started && s->state_gen == b->seen_gen[k]) continue;
+    if (s->crashed || (b->started && s->state_gen == b->seen_gen[k])) continue;
     fused_open(s);
     CHECK(fg_fused_prepare(s->ctx, s->d.DT - s->dtemp));
     fg_status st = fg_fused_results(s->ctx, &b->cfl[k], &b->force[2 * k], nullptr);
-    if (st) {
+    if (st == FG_E_CRASH) {
+      batch_crash(b, k, crash_msg, ncrash);
+    } else if (st) {
       const std::string why = fg_last_error();  // the message buffer is rewritten below
       host_error("batch disk %d: %s", k, why.c_str());
       return st;
@@ -1598,7 +1628,7 @@ extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) 
   for (int k = 0; k < n; k++) {
     fg_sim *s = b->sims[k];
     // a disk stops at the first DT boundary at or after t_end
-    const bool act = s->in_dt || s->PhysicalTime < t_end - 1e-9 * s->d.DT;
+    const bool act = !s->crashed && (s->in_dt || s->PhysicalTime < t_end - 1e-9 * s->d.DT);
     b->active[k] = act;
     if (!act) {
       CHECK(fg_batch_dev_set(b->dev, k, 0, 0, 0, 0, nullptr, 0, 0, 0, 0));
@@ -1612,7 +1642,7 @@ extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) 
                            L.DeltaT_next));
   }
   if (n_active) *n_active = na;
-  if (!na) return FG_OK;
+  if (!na) return batch_crash_status(crash_msg, ncrash);
   CHECK(fg_batch_dev_launch(b->dev));
   CHECK(fg_batch_dev_wait(b->dev));
   for (int k = 0; k < n; k++) {
@@ -1625,24 +1655,33 @@ extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) 
   for (int k = 0; k < n; k++) {
     if (!b->active[k]) continue;
     fg_status st = fg_fused_check(b->sims[k]->ctx, (double)b->cfl[k], &b->force[2 * k]);
-    if (st) {
-      const std::string why = fg_last_error();  // the message buffer is rewritten below
-      host_error("batch disk %d: %s", k, why.c_str());
-      return st;
-    }
+    if (st) batch_crash(b, k, crash_msg, ncrash);
   }
-  return FG_OK;
+  return batch_crash_status(crash_msg, ncrash);
 }
 
+// batch substeps until every disk that has not crashed reached t_end;
+// FG_E_CRASH at the end when disks crashed on the way (the others finished)
 extern "C" fg_status fg_batch_run(fg_batch *b, double t_end, long *substeps) {
   if (!b) return FG_E_INVALID;
   long count = 0;
   int na = 1;
+  std::string crash_msg;
   while (na) {
-    CHECK(fg_batch_substep(b, t_end, &na));
+    fg_status st = fg_batch_substep(b, t_end, &na);
+    if (st == FG_E_CRASH) {
+      if (!crash_msg.empty()) crash_msg += "; ";
+      crash_msg += fg_last_error();
+    } else if (st) {
+      return st;
+    }
     if (na) count++;
   }
   if (substeps) *substeps = count;
+  if (!crash_msg.empty()) {
+    host_error("%s", crash_msg.substr(0, 900).c_str());
+    return FG_E_CRASH;
+  }
   return FG_OK;
 }
 

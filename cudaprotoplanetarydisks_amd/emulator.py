@@ -100,7 +100,10 @@ def open_ensemble(params, directory, tiles=None):
 
 def generate(params, orbits, chunk=128, log=None):
     """Run every disk `orbits` orbits from the plain initial state (batched,
-    `chunk` disks at a time); returns the output vectors."""
+    `chunk` disks at a time); returns the output vectors, a row of NaN for
+    a disk whose run crashed (FG_E_CRASH: the batch stops that disk and
+    finishes the others)."""
+    from .native import FG_E_CRASH, FargoError
     out = []
     t_end = orbits * ORBIT
     for c0 in range(0, len(params), chunk):
@@ -108,10 +111,21 @@ def generate(params, orbits, chunk=128, log=None):
         with tempfile.TemporaryDirectory() as tmp:
             t0 = time.perf_counter()
             with open_ensemble(ps, tmp) as ens:
-                steps = ens.run(t_end)
-                out.extend(to_output(ens, k, p) for k, p in enumerate(ps))
+                try:
+                    steps = ens.run(t_end)
+                except FargoError as e:
+                    if e.status != FG_E_CRASH:
+                        raise
+                    steps = None
+                for k, p in enumerate(ps):
+                    if ens.get(k, "crashed"):
+                        out.append(np.full(2 * NOUT, np.nan, np.float32))
+                    else:
+                        out.append(to_output(ens, k, p))
             if log:
-                log({"generated": c0 + len(ps), "batch_substeps": steps, "seconds": round(time.perf_counter() - t0, 2)})
+                log({"generated": c0 + len(ps), "batch_substeps": steps,
+                     "crashed": [c0 + k for k in range(len(ps)) if not np.isfinite(out[c0 + k]).all()],
+                     "seconds": round(time.perf_counter() - t0, 2)})
     return np.stack(out)
 
 
@@ -136,8 +150,8 @@ class Emulator:
     def predict(self, x):
         return self.net.forward(x) * self.std + self.mean
 
-    def save(self, path):
-        np.savez(path, params=self.net.params().astype(np.float32), mean=self.mean, std=self.std)
+    def save(self, path, **meta):
+        np.savez(path, params=self.net.params().astype(np.float32), mean=self.mean, std=self.std, **meta)
 
     def load(self, path):
         z = np.load(path)
@@ -177,7 +191,10 @@ def orbit_history(ens, max_orbits, log=None, label=""):
     """Advance every disk max_orbits orbits; per orbit, the orbit average
     (over its 20 DT boundaries) of each disk's ring-averaged Sigma / Sigma_init
     and its steadiness against the previous orbit.  Returns R[disk][orbit],
-    the orbit averages hist[disk][orbit - 1] and the wall time."""
+    the orbit averages hist[disk][orbit - 1] and the wall time.  A disk
+    whose run crashes (the batch stops it and goes on with the others) gets
+    NaN steadiness from then on, so it never counts as steady."""
+    from .native import FG_E_CRASH, FargoError
     n = ens.n
     per_orbit = int(round(ORBIT / DT))
     rmed = [ens.ring(k, "Rmed")[:ens.shapes[k][0] - 1] for k in range(n)]
@@ -191,19 +208,24 @@ def orbit_history(ens, max_orbits, log=None, label=""):
     for o in range(1, max_orbits + 1):
         acc = [0.0] * n
         for q in range(1, per_orbit + 1):
-            ens.run(t_start + ((o - 1) * per_orbit + q) * DT)
+            try:
+                ens.run(t_start + ((o - 1) * per_orbit + q) * DT)
+            except FargoError as e:
+                if e.status != FG_E_CRASH:
+                    raise
             for k in range(n):
                 acc[k] = acc[k] + ens.dens(k)[:-1].mean(axis=1) / sigmed[k]
+        crashed = [bool(ens.get(k, "crashed")) for k in range(n)]
         avg = [a / per_orbit for a in acc]
         for k in range(n):
             hist[k].append(avg[k])
         if prev is not None:
             for k in range(n):
-                R[k].append(steadiness(prev[k], avg[k], rmed[k]))
+                R[k].append(float("nan") if crashed[k] else steadiness(prev[k], avg[k], rmed[k]))
         prev = avg
         if log and o % 25 == 0:
-            log({"start": label, "orbit": o, "R_median": float(np.median([r[-1] for r in R])),
-                 "seconds": round(time.perf_counter() - t0, 1)})
+            log({"start": label, "orbit": o, "R_median": float(np.nanmedian([r[-1] for r in R])),
+                 "crashed": int(sum(crashed)), "seconds": round(time.perf_counter() - t0, 1)})
     return R, hist, time.perf_counter() - t0
 
 
@@ -224,25 +246,39 @@ def study(n_train, n_eval, nrad, orbits, steps, eps, max_orbits, seed, log=print
     t0 = time.perf_counter()
     y_train = generate(train_p, orbits, log=log)
     report["generate_seconds"] = round(time.perf_counter() - t0, 1)
+    keep = np.isfinite(y_train).all(axis=1)  # runs that crashed are left out
+    report["n_train_crashed"] = int((~keep).sum())
     x_train = features(train_p)
     emu = Emulator(max_batch=max(n_train, n_eval))
     t0 = time.perf_counter()
-    losses = emu.fit(x_train, y_train, steps=steps)
+    losses = emu.fit(x_train[keep], y_train[keep], steps=steps)
     report["train_seconds"] = round(time.perf_counter() - t0, 2)
     report["loss_first_last"] = [float(losses[0]), float(losses[-1])]
     # held-out: profile error of the prediction vs the solver at `orbits`,
     # against the plain initial state as the prediction
     y_eval = generate(eval_p, orbits)
     pred = emu.predict(features(eval_p))
-    err_emu = np.abs(pred[:, :NOUT] - y_eval[:, :NOUT]).max(axis=1)
-    err_flat = np.abs(y_eval[:, :NOUT]).max(axis=1)
+    ok = np.isfinite(y_eval).all(axis=1)
+    err_emu = np.abs(pred[ok, :NOUT] - y_eval[ok, :NOUT]).max(axis=1)
+    err_flat = np.abs(y_eval[ok, :NOUT]).max(axis=1)
     report["sigma_err_at_train_orbits"] = {"emulator_median": float(np.median(err_emu)),
                                            "plain_ic_median": float(np.median(err_flat))}
-    # steadiness histories from cold and warm starts (the warm start goes
-    # through fg_sim_set_state, the restart path)
+    report.update(compare_starts(eval_p, pred, max_orbits, eps, log, keep_histories=True))
+    emu.close()
+    return report
+
+
+def compare_starts(eval_p, pred, max_orbits, eps, log=None, keep_histories=False):
+    """Cold start (the plain initial state) against warm start (the
+    emulator's prediction pred[k] set through fg_sim_set_state, the restart
+    path) for the disks eval_p, both batched for max_orbits orbits.  Per
+    steadiness threshold: the first orbit each start meets it, the speed-up,
+    and the profiles compared (mass-weighted L1 of the orbit-averaged ring
+    profile) where each start first meets it and at the end of the window."""
     import ctypes
     from .native import call
     _pd = ctypes.POINTER(ctypes.c_double)
+    n_eval = len(eval_p)
     with tempfile.TemporaryDirectory() as tmp:
         with open_ensemble(eval_p, tmp) as cold:
             R_cold, f_cold, wall_cold = orbit_history(cold, max_orbits, log, "cold")
@@ -253,22 +289,28 @@ def study(n_train, n_eval, nrad, orbits, steps, eps, max_orbits, seed, log=print
                 call("fg_sim_set_state", warm.sim_handle(k), d.ctypes.data_as(_pd), vr.ctypes.data_as(_pd),
                      vt.ctypes.data_as(_pd), None)
             R_warm, f_warm, wall_warm = orbit_history(warm, max_orbits, log, "warm")
-    report["wall_seconds_per_start"] = {"cold": round(wall_cold, 2), "warm": round(wall_warm, 2)}
-    report["steadiness_cold"] = [[round(x, 7) for x in r] for r in R_cold]
-    report["steadiness_warm"] = [[round(x, 7) for x in r] for r in R_warm]
+    report = {"wall_seconds_per_start": {"cold": round(wall_cold, 2), "warm": round(wall_warm, 2)}}
+    if keep_histories:
+        report["steadiness_cold"] = [[round(x, 7) for x in r] for r in R_cold]
+        report["steadiness_warm"] = [[round(x, 7) for x in r] for r in R_warm]
+
+    def l1(a, b):
+        return float(np.abs(a - b).sum() / np.abs(b).sum())
+
     per_eps = {}
     for e in eps:
         tc = [first_below(r, e) for r in R_cold]
         tw = [first_below(r, e) for r in R_warm]
         both = [k for k in range(n_eval) if tc[k] and tw[k]]
         # profiles at the orbit where each start first meets the criterion
-        steady_diff = []
-        for k in both:
-            pc, pw = f_cold[k][tc[k] - 1], f_warm[k][tw[k] - 1]
-            steady_diff.append(float(np.abs(pw - pc).sum() / np.abs(pc).sum()))
+        steady_diff = [l1(f_warm[k][tw[k] - 1], f_cold[k][tc[k] - 1]) for k in both]
+        # and at the end of the window, for the disks where both met it
+        end_diff = [l1(f_warm[k][-1], f_cold[k][-1]) for k in both]
         per_eps[str(e)] = {
             "steady_profile_l1_median": float(np.median(steady_diff)) if steady_diff else None,
             "steady_profile_within_1pct": int(sum(x <= 0.01 for x in steady_diff)),
+            "end_profile_l1_median": float(np.median(end_diff)) if end_diff else None,
+            "end_profile_within_1pct": int(sum(x <= 0.01 for x in end_diff)),
             "reached_cold": int(sum(t is not None for t in tc)), "reached_warm": int(sum(t is not None for t in tw)),
             "orbits_cold": tc, "orbits_warm": tw,
             "median_speedup": float(np.median([tc[k] / tw[k] for k in both])) if both else None,
@@ -276,22 +318,78 @@ def study(n_train, n_eval, nrad, orbits, steps, eps, max_orbits, seed, log=print
             "total_orbits_warm": int(sum(t if t else max_orbits for t in tw)),
         }
     report["time_to_steady"] = per_eps
-    diffs = []
-    for k in range(n_eval):
-        d = np.abs(f_warm[k][-1] - f_cold[k][-1])
-        diffs.append({"l1": float(d.sum() / np.abs(f_cold[k][-1]).sum()),
-                      "linf": float((d / np.abs(f_cold[k][-1])).max())})
-    report["final_profile_diff"] = {"l1_median": float(np.median([x["l1"] for x in diffs])),
-                                    "l1_max": float(max(x["l1"] for x in diffs)),
-                                    "within_1pct_l1": int(sum(x["l1"] <= 0.01 for x in diffs)),
-                                    "linf_median": float(np.median([x["linf"] for x in diffs]))}
-    emu.close()
+    alive = [k for k in range(n_eval) if np.isfinite(R_cold[k][-1]) and np.isfinite(R_warm[k][-1])]
+    report["crashed"] = {"cold": int(sum(not np.isfinite(r[-1]) for r in R_cold)),
+                         "warm": int(sum(not np.isfinite(r[-1]) for r in R_warm))}
+    diffs = [l1(f_warm[k][-1], f_cold[k][-1]) for k in alive]
+    report["final_profile_diff"] = {"l1_median": float(np.median(diffs)) if diffs else None,
+                                    "l1_max": float(max(diffs)) if diffs else None,
+                                    "within_1pct_l1": int(sum(x <= 0.01 for x in diffs))}
     return report
+
+
+def train(n_train, nrad, orbits, steps, seed, out, log=None):
+    """Train the emulator on `n_train` batched runs of `orbits` orbits at
+    nrad x 3 nrad (draws of `seed`) and save its weights with that setup:
+    the file the warm-start leg of bench.py loads."""
+    params = draw(n_train, seed, nrad)
+    t0 = time.perf_counter()
+    y = generate(params, orbits, log=log)
+    gen_s = time.perf_counter() - t0
+    keep = np.isfinite(y).all(axis=1)  # disks whose runs crashed are left out
+    emu = Emulator(max_batch=n_train)
+    try:
+        t0 = time.perf_counter()
+        losses = emu.fit(features(params)[keep], y[keep], steps=steps)
+        fit_s = time.perf_counter() - t0
+        emu.save(out, nrad=nrad, orbits=orbits, seed=seed, n_train=n_train, n_crashed=int((~keep).sum()), steps=steps,
+                 widths=np.array(WIDTHS), loss_first_last=np.array([losses[0], losses[-1]]),
+                 generate_seconds=gen_s, train_seconds=fit_s)
+    finally:
+        emu.close()
+    return {"out": out, "n_train": n_train, "n_crashed": int((~keep).sum()),
+            "generate_seconds": round(gen_s, 1), "train_seconds": round(fit_s, 2),
+            "loss_first_last": [float(losses[0]), float(losses[-1])]}
+
+
+def warm_start_leg(weights, n_eval, max_orbits, eps, log=None):
+    """The ensemble time-to-steady of held-out draws (the training seed + 1),
+    emulator warm start against plain cold start, with committed weights."""
+    z = np.load(weights)
+    nrad, seed = int(z["nrad"]), int(z["seed"])
+    eval_p = draw(n_eval, seed + 1, nrad)
+    emu = Emulator(max_batch=max(n_eval, 8))
+    try:
+        emu.load(weights)
+        pred = emu.predict(features(eval_p))
+    finally:
+        emu.close()
+    rep = {"grid": [nrad, 3 * nrad], "n_eval": n_eval, "eval_seed": seed + 1, "max_orbits": max_orbits,
+           "trained_on": {"n_train": int(z["n_train"]), "orbits": float(z["orbits"]), "seed": seed},
+           "weights": os.path.basename(weights)}
+    rep.update(compare_starts(eval_p, pred, max_orbits, eps, log))
+    return rep
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+WEIGHTS = os.path.join(DATA, "emulator_128x384.npz")
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="emulator warm-start study")
     sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--n-train", type=int, default=512)
+    t.add_argument("--nrad", type=int, default=128)
+    t.add_argument("--orbits", type=float, default=150.0)
+    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--seed", type=int, default=20261202)
+    t.add_argument("--out", default=WEIGHTS)
+    w = sub.add_parser("warmstart")
+    w.add_argument("--weights", default=WEIGHTS)
+    w.add_argument("--n-eval", type=int, default=16)
+    w.add_argument("--max-orbits", type=int, default=100)
+    w.add_argument("--eps", type=float, nargs="+", default=[2e-3, 1e-3, 5e-4, 2e-4])
     s = sub.add_parser("study")
     s.add_argument("--n-train", type=int, default=512)
     s.add_argument("--n-eval", type=int, default=32)
@@ -307,6 +405,12 @@ def main(argv=None):
     def log(d):
         print(json.dumps(d), flush=True)
 
+    if args.cmd == "train":
+        log(train(args.n_train, args.nrad, args.orbits, args.steps, args.seed, args.out, log))
+        return 0
+    if args.cmd == "warmstart":
+        log(warm_start_leg(args.weights, args.n_eval, args.max_orbits, args.eps, log))
+        return 0
     rep = study(args.n_train, args.n_eval, args.nrad, args.orbits, args.steps, args.eps, args.max_orbits,
                 args.seed, log)
     log(rep)

@@ -294,7 +294,7 @@ fg_status fg_sim_set_state(fg_sim *sim, const double *dens, const double *vrad, 
                            const double *energy);
 /* scalars: "PhysicalTime", "OmegaFrame", "LostMass", "dt", "dtemp",
  * "MassTaper", "DT", "NTOT", "NINTERM", "NRAD", "NSEC", "last_cfl",
- * "mdcp", "mdcp1", "exces_mdcp" (diagnostics) */
+ * "mdcp", "mdcp1", "exces_mdcp" (diagnostics), "crashed" (batch disks) */
 fg_status fg_sim_get(fg_sim *sim, const char *name, double *value);
 fg_status fg_sim_nplanets(fg_sim *sim, int *nb);
 /* out = {x, y, vx, vy, mass} */
@@ -317,9 +317,13 @@ typedef struct fg_batch fg_batch;
 fg_status fg_batch_create(int n, const char *const *parfiles, const fg_sim_options *opt, fg_batch **out);
 fg_status fg_batch_destroy(fg_batch *b);
 fg_status fg_batch_set_tiles(fg_batch *b, int k, int src_rows, int trad_band);
-/* one batch substep; *n_active = disks that moved (0: all reached t_end) */
+/* one batch substep; *n_active = disks that moved (0: all reached t_end).
+ * A disk whose state blows up is marked crashed (fg_sim_get "crashed") and
+ * no longer advanced; the substep in which that happens returns
+ * FG_E_CRASH naming it, and the other disks go on. */
 fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active);
-/* batch substeps until every disk reached t_end */
+/* batch substeps until every disk that has not crashed reached t_end;
+ * FG_E_CRASH at the end when some crashed on the way */
 fg_status fg_batch_run(fg_batch *b, double t_end, long *substeps);
 /* disk k (state, scalars, planets through the fg_sim_* calls) */
 fg_sim *fg_batch_sim(fg_batch *b, int k);

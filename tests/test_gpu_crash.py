@@ -57,6 +57,16 @@ def test_crash_in_a_batch_disk(tmp_path):
             ens.substep(10.0)
         assert e.value.status == FG_E_CRASH
         assert "disk 1" in str(e.value)
+        # the crashed disk is left alone, the other one goes on
+        assert ens.get(1, "crashed") == 1.0 and ens.get(0, "crashed") == 0.0
+        t0 = ens.get(0, "PhysicalTime")
+        assert ens.substep(10.0) == 1
+        assert ens.get(0, "PhysicalTime") > t0
+        # a good state brings it back
+        dens[5, 3] = 1e-3
+        call("fg_sim_set_state", ens.sim_handle(1), *[a.ctypes.data_as(_pd) for a in (dens, vr, vt)], None)
+        assert ens.get(1, "crashed") == 0.0
+        assert ens.substep(10.0) == 2
 
 
 def test_state_set_between_batch_substeps_is_stepped_like_a_single_run(tmp_path):
