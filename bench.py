@@ -192,28 +192,35 @@ def small_grid_leg(steps, warmup):
 def adsg_leg(steps, warmup):
     """The ADSG workload (decks/c2_adsg.par: 512 x 1536, adiabatic with beta
     cooling, alpha viscosity, self-gravity on the 1024 x 1536 padded grid,
-    a planet feeling the disk) on the per-routine FP64 path: zone-updates/s
-    and the share of HBM peak at the compulsory 64 B/zone (Sigma, v_r,
-    v_theta, e read and written) + 576 B/zone of self-gravity FFT traffic."""
+    a planet feeling the disk) on the fused FP64 path, and on the
+    per-routine path for comparison: zone-updates/s and the share of HBM
+    peak at the compulsory 64 B/zone (Sigma, v_r, v_theta, e read and
+    written) + 576 B/zone of self-gravity FFT traffic."""
     from cudaprotoplanetarydisks_amd import Simulation
     from cudaprotoplanetarydisks_amd.native import call
-    with tempfile.TemporaryDirectory() as tmp:
-        with Simulation(deck_copy(tmp, "c2_adsg.par")) as sim:
-            ctx = sim.ctx.h
-            for _ in range(warmup):
-                sim.substep()
-            call("fg_sync", ctx)
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                sim.substep()
-            call("fg_sync", ctx)
-            el = time.perf_counter() - t0
-            zones = sim.nrad * sim.nsec
-        zus = zones * steps / el
-        return {"workload": "ADSG + self-gravity, 512x1536 (decks/c2_adsg.par), per-routine FP64 path",
-                "steps": steps, "ms_per_step": round(1e3 * el / steps, 4), "value": round(zus, 1),
-                "unit": "zone-updates/s", "b_alg_per_zone": 64 + 576,
-                "pct_hbm_peak": round(100.0 * zus * (64 + 576) / (HBM_PEAK_GBS * 1e9), 3)}
+
+    def run(fused):
+        with tempfile.TemporaryDirectory() as tmp:
+            with Simulation(deck_copy(tmp, "c2_adsg.par"), fused=fused) as sim:
+                assert sim.get("fused_active") == (1.0 if fused else 0.0)
+                ctx = sim.ctx.h
+                for _ in range(warmup):
+                    sim.substep()
+                call("fg_sync", ctx)
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    sim.substep()
+                call("fg_sync", ctx)
+                el = time.perf_counter() - t0
+                return sim.nrad * sim.nsec * steps / el, el
+
+    zus, el = run(True)
+    rzus, rel = run(False)
+    return {"workload": "ADSG + self-gravity, 512x1536 (decks/c2_adsg.par), fused FP64 path",
+            "steps": steps, "ms_per_step": round(1e3 * el / steps, 4), "value": round(zus, 1),
+            "unit": "zone-updates/s", "b_alg_per_zone": 64 + 576,
+            "pct_hbm_peak": round(100.0 * zus * (64 + 576) / (HBM_PEAK_GBS * 1e9), 3),
+            "per_routine": {"ms_per_step": round(1e3 * rel / steps, 4), "value": round(rzus, 1)}}
 
 
 def emulator_leg(n_eval, max_orbits):
@@ -265,7 +272,7 @@ def main():
     ap.add_argument("--no-alt", action="store_true", help="skip the leg in the other precision")
     ap.add_argument("--ensemble-disks", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=100)
-    ap.add_argument("--adsg-steps", type=int, default=10, help="substeps of the ADSG leg (0: skip)")
+    ap.add_argument("--adsg-steps", type=int, default=50, help="substeps of the ADSG leg (0: skip)")
     ap.add_argument("--small-steps", type=int, default=1000, help="substeps of the 128x384 leg (0: skip)")
     ap.add_argument("--emulator-disks", type=int, default=16, help="held-out disks of the warm-start leg (0: skip)")
     ap.add_argument("--emulator-orbits", type=int, default=150)

@@ -186,6 +186,8 @@ extern "C" fg_status fg_create(const fg_params *p, const fg_tables *t, int devic
   TRY(dev_alloc(&c->nu, R));
   TRY(dev_alloc(&c->cs_mean, R));
   TRY(dev_alloc(&c->sgacc_mean, R));
+  TRY(dev_alloc(&c->sgacc_prev, R));
+  TRY(dev_alloc(&c->nu_next, R));
   TRY(dev_alloc(&c->Nshift, R));
   TRY(dev_alloc(&c->NoSplit, R));
   TRY(dev_alloc(&c->LostByDisk, NS));
@@ -237,6 +239,7 @@ extern "C" fg_status fg_destroy(fg_ctx *c) {
                   c->DRP,          c->Work,         c->QRStar,        c->DensStar,       c->dq,
                   c->TempShift,    c->Vradial,      c->Vazimutal,     c->Vmoy,           c->newDT,
                   c->DT1D,         c->VMed,         c->nu,            c->cs_mean,        c->sgacc_mean,
+                  c->sgacc_prev,   c->nu_next,
                   c->Nshift,       c->NoSplit,      c->LostByDisk,    c->red_partial,    c->red_out,
                   c->cfl_out,      c->sg_Sr,        c->sg_St,         c->sg_Krc,         c->sg_Ktc,
                   c->sg_Gr,        c->sg_Gt,        c->sg_tmp,        c->fu_ring,        c->fu_fpart,

@@ -50,12 +50,27 @@ struct fused_bufs {
   double *Rd, *Rrmp, *Rrmm, *Rtmp, *Rtmm;  // after k_trad
   double *Xd, *Xvt, *Xrmp, *Xrmm;        // after k_taz
   double *P;                             // pressure (temperature output only)
+  // adiabatic decks: EnergyInt after k_src, energy after k_trad and k_taz;
+  // self-gravity: the kick of VradInt / VthetaInt (NULL when not allocated)
+  double *Beint, *Re, *Xe, *kick_r, *kick_t;
 };
 
 struct src_args {
   const double *dens, *vr, *vt;
   double *odens, *ovr, *ovt, *opress;
-  const double *nu;
+  const double *nu;   // viscosity of this substep (viscous stress, second damping pass)
+  const double *nu1;  // viscosity as the first damping pass sees it (an adiabatic
+                      // run refreshes it after that pass on a DT's first substep)
+  // adiabatic: energy in, EnergyInt out (SourceEuler.cu:281, Kernels.cu:474-477)
+  const double *energy;
+  double *oeint;
+  double gamma;
+  // self-gravity: the kick of Update_sgvelocityKernel per cell (Kernels.cu:915-932,
+  // formed by k_sg_acc_fused), and the ring means of the radial SG acceleration
+  // that the first (sg1: previous evaluation) and second (sg2: this substep's)
+  // damping passes take for vtheta0
+  const double *kick_r, *kick_t;
+  const double *sg1, *sg2;
   fg_dev_tables t;
   fg_planets_arg P;
   double dt, omega_frame, domega;
@@ -85,18 +100,21 @@ __device__ __forceinline__ double cell_potential(const fg_planets_arg &P, double
   return acc;
 }
 
-// evanescent damping constants of ring i for a step (Kernels.cu:978-1004)
+// evanescent damping constants of ring i for a step (Kernels.cu:978-1004);
+// with self-gravity vtheta0 takes the ring mean of the radial SG
+// acceleration (sgacc, NULL without self-gravity)
 __device__ __forceinline__ void ev_consts(const fg_dev_tables &t, const double *nu, int i, double step,
-                                          double omega_frame, double &lambda, double &vrad0, double &vtheta0) {
+                                          double omega_frame, double &lambda, double &vrad0, double &vtheta0,
+                                          const double *sgacc = nullptr) {
   const double Rm = t.Rmed[i];
   lambda = t.ev_coef[i] * step / t.ev_period[i];
-  vtheta0 = sqrt(t.ev_vtheta_arg[i]);
+  vtheta0 = sgacc ? sqrt(t.ev_vtheta_arg[i] - Rm * sgacc[i]) : sqrt(t.ev_vtheta_arg[i]);
   vtheta0 -= Rm * omega_frame;
   vrad0 = -3.0 * nu[i] / Rm * t.vrad0_slope;
 }
 
 // per-ring tables the source stages read, staged per tile row in LDS
-enum { RT_MED, RT_IMED, RT_IDMED, RT_INF, RT_IINF, RT_IDXT, RT_SUPP, RT_IDSUP, RT_SUP, RT_NU, RT_CS, RT_SMED, RT_EVZ, RT_N };
+enum { RT_MED, RT_IMED, RT_IDMED, RT_INF, RT_IINF, RT_IDXT, RT_SUPP, RT_IDSUP, RT_SUP, RT_NU, RT_CS, RT_SMED, RT_EVZ, RT_EMED, RT_N };
 __device__ __forceinline__ const double *ring_table(const fg_dev_tables &t, const double *nu, int x) {
   switch (x) {
     case RT_MED: return t.Rmed;
@@ -114,8 +132,11 @@ __device__ __forceinline__ const double *ring_table(const fg_dev_tables &t, cons
   }
 }
 
-template <int KTH, int KTW, int KTPB>
+// MODE bit 1: adiabatic (energy damped, P = (gamma - 1) e, EnergyInt out);
+// bit 2: self-gravity kick after Substep1
+template <int KTH, int KTW, int KTPB, int MODE>
 __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
+  constexpr bool ADI = MODE & 1, SGK = MODE & 2;
   constexpr int KHR = KTH + HALO_R + HALO_B, KHC = KTW + 2 * HALO_C;
   constexpr int KPT = (KHR * KHC + KTPB - 1) / KTPB;
   __shared__ double sS[KHR][KHC];   // Sigma after the first damping
@@ -134,11 +155,13 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
 
   // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
   // all of a thread's loads are issued before any of them is used
-  double ls[KPT], lvr[KPT], lvt[KPT];
+  double ls[KPT], lvr[KPT], lvt[KPT], le[ADI ? KPT : 1], lkr[SGK ? KPT : 1], lkt[SGK ? KPT : 1];
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
     const int q = tid + k * KTPB;
     ls[k] = lvr[k] = lvt[k] = 0.0;
+    if constexpr (ADI) le[k] = 0.0;
+    if constexpr (SGK) lkr[k] = lkt[k] = 0.0;
     if (q < KHR * KHC) {
       const int r = q / KHC, cc = q - r * KHC, i = GI(r);
       if (i >= 0 && i <= nr) {
@@ -147,6 +170,12 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
         ls[k] = a.dens[l];
         lvr[k] = a.vr[l];
         lvt[k] = a.vt[l];
+        if constexpr (ADI) le[k] = a.energy[l];
+        if constexpr (SGK)
+          if (i < nr) {
+            lkr[k] = a.kick_r[l];
+            lkt[k] = a.kick_t[l];
+          }
       }
     }
   }
@@ -171,6 +200,7 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
       rt[w + 4][ln] = v1;
       rt[w + 8][ln] = v2;
       if (w == 0) rt[RT_EVZ][ln] = (double)ez;
+      if (ADI && w == 1) rt[RT_EMED][ln] = t.EnergyMed[i];
     }
     for (int c = tid; c < KHC; c += KTPB) {
       const int j = GJ(c);
@@ -178,12 +208,16 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
       ct[1][c] = t.sin_pot[j];
     }
   }
-  // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004)
-  __shared__ double evc[KHR][3];
+  // damping constants of the tile's rings, once per ring (Kernels.cu:978-1004):
+  // lambda, then vrad0 and vtheta0 of the first pass, then of the second
+  __shared__ double evc[KHR][5];
   if (a.evanescent) {
     for (int r = tid; r < KHR; r += KTPB) {
       const int i = GI(r);
-      if (i >= 0 && i < nr && t.ev_zone[i]) ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2]);
+      if (i >= 0 && i < nr && t.ev_zone[i]) {
+        ev_consts(t, a.nu1, i, dt, a.omega_frame, evc[r][0], evc[r][1], evc[r][2], a.sg1);
+        ev_consts(t, a.nu, i, dt, a.omega_frame, evc[r][0], evc[r][3], evc[r][4], a.sg2);
+      }
     }
   }
   __syncthreads();
@@ -200,13 +234,19 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
       vt = lvt[k];
       if (i < nr) {
         if (a.corotating) vt = vt - a.domega * rt[RT_MED][r];
+        double e = ADI ? le[k] : 0.0;
         if (a.evanescent && rt[RT_EVZ][r] != 0.0) {
           vr = damp_toward(vr, evc[r][0], evc[r][1]);
           vt = damp_toward(vt, evc[r][0], evc[r][2]);
           s = damp_toward(s, evc[r][0], rt[RT_SMED][r]);
+          if (ADI) e = damp_toward(e, evc[r][0], rt[RT_EMED][r]);
         }
-        const double cs = rt[RT_CS][r];
-        p = s * cs * cs;
+        if (ADI) {
+          p = (a.gamma - 1.0) * e;  // Kernels.cu:196
+        } else {
+          const double cs = rt[RT_CS][r];
+          p = s * cs * cs;
+        }
         phi = cell_potential(a.P, rt[RT_MED][r], rt[RT_IMED][r], ct[0][cc], ct[1][cc], a.indirect);
       }
     }
@@ -255,8 +295,9 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
         v += dt * rt[RT_SUPP][r];
         vti = v;
       }
-      regA[k] = vri;
-      regB[k] = vti;
+      // self-gravity kick on VradInt / VthetaInt (0 on ring 0 and outside)
+      regA[k] = SGK ? vri + lkr[k] : vri;
+      regB[k] = SGK ? vti + lkt[k] : vti;
     }
     if (a.opress) {  // pressure for the temperature output, own cells only
       for (int q = tid; q < KTH * KTW; q += KTPB) {
@@ -365,9 +406,18 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
     double tempint = (sD[r][cc] - sD[r][cc - 1]);
     double vtnew = sB[r][cc] - dt * 2.0 / dens2 * tempint * invdxtheta;
     double s = sS[r][cc];
+    if constexpr (ADI) {
+      // EnergyInt ("kernel", Kernels.cu:474-477) from the energy after the
+      // first damping pass; the second pass damps Energy, which Substep3
+      // then overwrites, so it is not formed
+      double e = a.energy[l];
+      if (a.evanescent && rt[RT_EVZ][r] != 0.0) e = damp_toward(e, evc[r][0], rt[RT_EMED][r]);
+      a.oeint[l] = e - dt * sC[r][cc] * (sA[r + 1][cc] - sA[r][cc]) * rt[RT_IDSUP][r] -
+                   dt * sD[r][cc] * (sB[r][cc + 1] - sB[r][cc]) * invdxtheta;
+    }
     if (a.evanescent && rt[RT_EVZ][r] != 0.0) {
-      vrnew = damp_toward(vrnew, evc[r][0], evc[r][1]);
-      vtnew = damp_toward(vtnew, evc[r][0], evc[r][2]);
+      vrnew = damp_toward(vrnew, evc[r][0], evc[r][3]);
+      vtnew = damp_toward(vtnew, evc[r][0], evc[r][4]);
       s = damp_toward(s, evc[r][0], rt[RT_SMED][r]);
     }
     a.odens[l] = s;
@@ -376,6 +426,82 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
   }
 #undef GI
 #undef GJ
+}
+
+// ----------------------------------------------------------- k_src3 ----
+// Substep3 of an adiabatic deck (SourceEuler.cu:287-291): the viscous
+// terms of the velocities after the second damping pass (ViscousTermsKernel,
+// Kernels.cu:733-771), Qplus (Substep3Kernel, Kernels.cu:56-77; ring 0 by
+// the power-law extrapolation from rings 1 and 2, Kernels.cu:88-95) and
+// the implicit energy update with optional relaxation cooling
+// (Substep3Kernel2, Kernels.cu:97-113).  One thread per cell; the
+// neighbours are read through the caches.
+struct src3_args {
+  const double *dens, *vr, *vt, *eint;
+  double *energy;
+  const double *nu;
+  fg_dev_tables t;
+  double dt, gamma, lr01, lr12;
+  int nr, ns, cooling;
+};
+
+// divV of cell (i, j) and, for i >= 1, its Qplus
+__device__ __forceinline__ void src3_cell(const src3_args &a, int i, int j, double &qplus, double &divv) {
+  const fg_dev_tables &t = a.t;
+  const int ns = a.ns;
+  const long l = (long)i * ns + j;
+  const long JP = (long)i * ns + (j + 1 == ns ? 0 : j + 1), JM = (long)i * ns + (j == 0 ? ns - 1 : j - 1);
+  const double invdphi = t.invdphi;
+  const double onethird = 1.0 / 3.0;
+  const double vrl = a.vr[l], vrn = a.vr[l + ns], vtl = a.vt[l], vtp = a.vt[JP];
+  const double drr = (vrn - vrl) * t.invdiffRsup[i];
+  const double dpp = (vtp - vtl) * invdphi * t.invRmed[i] + 0.5 * (vrn + vrl) * t.invRmed[i];
+  double dv = (vrn * t.Rsup[i] - vrl * t.Rinf[i]) * t.invdiffRsup[i] * t.invRmed[i];
+  dv += (vtp - vtl) * invdphi * t.invRmed[i];
+  divv = dv;
+  qplus = 0.0;
+  if (i == 0) return;
+  const double d = a.dens[l], nu = a.nu[i];
+  if (nu == 0.0) return;
+  const double TRR = 2.0 * d * nu * (drr - onethird * dv);
+  const double TPP = 2.0 * d * nu * (dpp - onethird * dv);
+  const double drp = 0.5 * (t.Rinf[i] * (vtl * t.invRmed[i] - a.vt[l - ns] * t.invRmed[i - 1]) * t.invdiffRmed[i] +
+                            (vrl - a.vr[JM]) * invdphi * t.invRinf[i]);
+  const double TRP = 2.0 * 0.25 * (d + a.dens[l - ns] + a.dens[JM] + a.dens[JM - ns]) * nu * drp;
+  double q = 0.5 / nu / d * (TRR * TRR + TRP * TRP + TPP * TPP);
+  q += (2.0 / 9.0) * nu * d * dv * dv;
+  qplus = q;
+}
+
+__global__ __launch_bounds__(256) void k_src3(src3_args a) {
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= a.nr || j >= a.ns) return;
+  const fg_dev_tables &t = a.t;
+  const long l = (long)i * a.ns + j;
+  double qplus, divv;
+  src3_cell(a, i, j, qplus, divv);
+  if (i == 0) {
+    if (a.nu[a.nr - 1] != 0.0) {
+      double q1, q2, unused;
+      src3_cell(a, 1, j, q1, unused);
+      src3_cell(a, 2, j, q2, unused);
+      qplus = q1 * exp(log(q1 / q2) * a.lr01 / a.lr12);
+    } else {
+      qplus = 0.0;
+    }
+  }
+  double num, den;
+  if (!a.cooling) {
+    num = a.dt * qplus + a.eint[l];
+    den = 1.0 + (a.gamma - 1.0) * a.dt * divv;
+  } else {
+    const double d = a.dens[l];
+    num = t.EnergyMed[i] * a.dt * d / t.SigmaMed[i] + t.CoolingTimeMed[i] * a.eint[l] +
+          a.dt * t.CoolingTimeMed[i] * (qplus - t.QplusMed[i] * d / t.SigmaMed[i]);
+    den = a.dt + t.CoolingTimeMed[i] + (a.gamma - 1.0) * a.dt * t.CoolingTimeMed[i] * divv;
+  }
+  a.energy[l] = num / den;
 }
 
 // ----------------------------------------------------------- k_trad ----
@@ -387,10 +513,12 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
 // neighbouring rings (Rsup[k-1] == Rinf[k]).
 #define TRAD_TPB 64
 #define NQ 5  // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Sigma
+#define NQ_MAX 6  // adiabatic: RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Sigma
 
 struct trad_args {
   const double *dens, *vr, *vt;
-  double *out[NQ];
+  const double *energy;  // adiabatic decks (NQT = 6)
+  double *out[NQ_MAX];
   double *vmed;  // ring means of vtheta for the azimuthal pass
   fg_dev_tables t;
   double dt, omega_frame;
@@ -460,7 +588,11 @@ struct ring_lanes {
   }
 };
 
+// NQT transported quantities: 5, or 6 with the energy of an adiabatic deck
+// (TransportEuler.cu:60-79 sweeps it between the momenta and Sigma)
+template <int NQT>
 __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
+  constexpr bool EN = NQT == 6;
   const int nr = a.nr, ns = a.ns;
   // blocks [0, NRAD): one ring mean each, issued first so that these
   // latency-bound sums overlap the sweep blocks
@@ -483,7 +615,7 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   // slope) and face x (its flux) live in slot x % 3, so a step writes its
   // new ring over the one three back and nothing is shifted; the step for
   // ring m is instantiated per m % 3 to keep the slots static
-  double Q[NQ][3] = {}, W[NQ][3] = {}, dq[NQ][3] = {}, F[NQ][3] = {};
+  double Q[NQT][3] = {}, W[NQT][3] = {}, dq[NQT][3] = {}, F[NQT][3] = {};
   double sg[3] = {}, dqs[3] = {}, vr[3] = {};
   const int m0 = max(b0 - 2, 0);
   ring_lanes rt;
@@ -492,7 +624,7 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   // the loads are unconditional, so that every iteration of a loop issues
   // the same memory operations and the waits for them can count past the
   // stores); vrad of ring m + 1 is carried over as the next ring's own
-  double n_s, n_v0, n_v1, n_vtp, n_vt;
+  double n_s, n_v0, n_v1, n_vtp, n_vt, n_e = 0.0;
   {
     const long l = (long)m0 * ns + j;
     n_s = a.dens[l];
@@ -500,7 +632,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     n_v1 = a.vr[l + ns];
     n_vtp = a.vt[(long)m0 * ns + jp];
     n_vt = a.vt[l];
-    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+    if constexpr (EN) n_e = a.energy[l];
+    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt), "v"(n_e));
   }
   // inner: the caller guarantees 2 <= m < nr (ring m inside the disk, slope
   // and face of ring m - 1 interior), which drops the edge cases
@@ -508,7 +641,7 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     constexpr int C = decltype(phase)::value, B1 = (C + 2) % 3, B2 = (C + 1) % 3;  // rings m, m-1, m-2
     constexpr bool IN = decltype(inner)::value;
     const int r = rt.at(t, m, nr);
-    const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
+    const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt, c_e = n_e;
     {
       const long l = (long)min(m + 1, nr - 1) * ns;
       n_s = a.dens[l + j];
@@ -516,6 +649,7 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
       n_v1 = a.vr[l + ns + j];
       n_vtp = a.vt[l + jp];
       n_vt = a.vt[l + j];
+      if constexpr (EN) n_e = a.energy[l + j];
     }
     // ring tables: Rmed[m], Rmed[m-1], Rmed[m-2] at lanes r, r-1, r-2
     const double Rm = lane_get(rt.Rmed, r);
@@ -526,13 +660,14 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
       Q[1][C] = s * v0;
       Q[2][C] = s * (c_vtp + Rm * OF) * Rm;
       Q[3][C] = s * (c_vt + Rm * OF) * Rm;
-      Q[4][C] = s;
+      if constexpr (EN) Q[4][C] = c_e;
+      Q[NQT - 1][C] = s;
       sg[C] = s;
       vr[C] = v0;
       const recip64 rs(s + 1e-20);  // Work = Q / (Sigma + 1e-20), one denominator
-      for (int q = 0; q < NQ; q++) W[q][C] = rs.div(Q[q][C]);
+      for (int q = 0; q < NQT; q++) W[q][C] = rs.div(Q[q][C]);
     } else {
-      for (int q = 0; q < NQ; q++) Q[q][C] = W[q][C] = 0.0;
+      for (int q = 0; q < NQT; q++) Q[q][C] = W[q][C] = 0.0;
       sg[C] = 0.0;
       vr[C] = 0.0;  // vrad of the ghost ring
     }
@@ -541,29 +676,29 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
     if (!decltype(store)::value && k < 0) return;  // storing steps have k >= 1
     const bool flat = !IN && (k == 0 || k >= nr - 1);
     const double idm0 = lane_get(rt.idm, r - 1), idm1 = lane_get(rt.idm, r);
-    for (int q = 0; q < NQ; q++) dq[q][B1] = flat ? 0.0 : vl_rad_slope(W[q][B2], W[q][B1], W[q][C], idm0, idm1);
+    for (int q = 0; q < NQT; q++) dq[q][B1] = flat ? 0.0 : vl_rad_slope(W[q][B2], W[q][B1], W[q][C], idm0, idm1);
     dqs[B1] = flat ? 0.0 : vl_rad_slope(sg[B2], sg[B1], sg[C], idm0, idm1);
     // face k: upwind values (StarRadKernel2); faces 0 and NRAD are zero
     double DS = 0.0;
-    double QS[NQ];
+    double QS[NQT];
     if (IN || (k > 0 && k < nr)) {
       const double vk = vr[B1];
       const double Rkm = lane_get(rt.Rmed, r - 2), Rk = lane_get(rt.Rmed, r - 1);
       DS = vl_rad_star(vk, dt, sg[B2], sg[B1], dqs[B2], dqs[B1], Rkm, Rk, Rm, k);
-      for (int q = 0; q < NQ; q++)
+      for (int q = 0; q < NQT; q++)
         QS[q] = vl_rad_star(vk, dt, W[q][B2], W[q][B1], dq[q][B2], dq[q][B1], Rkm, Rk, Rm, k);
     } else {
-      for (int q = 0; q < NQ; q++) QS[q] = 0.0;
+      for (int q = 0; q < NQT; q++) QS[q] = 0.0;
     }
     const double vface = (IN || k < nr) ? vr[B1] : 0.0;
     const double Rinf = lane_get(rt.Rinf, r - 1);
-    for (int q = 0; q < NQ; q++) F[q][B1] = cflux * Rinf * QS[q] * DS * vface;
+    for (int q = 0; q < NQT; q++) F[q][B1] = cflux * Rinf * QS[q] * DS * vface;
     // ring k-1 is complete: Q += (F(k-1) - F(k)) / Surf
     if (decltype(store)::value) {
       const int o = k - 1;
       const long lo = (long)o * ns + j;
       const double isurf = lane_get(rt.isurf, r - 2);
-      for (int q = 0; q < NQ; q++) {
+      for (int q = 0; q < NQT; q++) {
         double varq = F[q][B2];
         varq -= F[q][B1];
         a.out[q][lo] = Q[q][B2] + varq * isurf;
@@ -574,7 +709,7 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   // every window by one, then the step of phase 2.  Three consecutive
   // steps of phases 0, 1, 2 leave the windows in that same layout.
   auto shift = [&]() {
-    for (int q = 0; q < NQ; q++) {
+    for (int q = 0; q < NQT; q++) {
       Q[q][0] = Q[q][1];
       Q[q][1] = Q[q][2];
       W[q][0] = W[q][1];
@@ -602,14 +737,14 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
   }
   // the storing loop starts with nothing in flight from the filling one;
   // its steps have m >= b0 + 2 >= 2, so those below nr are inner
-  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt), "v"(n_e));
   const int mi = min(b1 + 1, nr - 1);
   for (; m + 2 <= mi; m += 3) {
     ring(m, T(), P0(), T());
     ring(m + 1, T(), P1(), T());
     ring(m + 2, T(), P2(), T());
   }
-  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt), "v"(n_e));
   for (; m <= b1 + 1; m++) {
     shift();
     ring(m, T(), P2(), F0());
@@ -624,8 +759,8 @@ __global__ __launch_bounds__(TRAD_TPB) void k_trad(trad_args a) {
 struct taz_args {
   const double *vt;     // vtheta after k_src
   const double *vmed;   // its ring means
-  const double *q[NQ];  // quantities after the radial sweep
-  double *xd, *xvt, *xrmp, *xrmm;
+  const double *q[NQ_MAX];  // quantities after the radial sweep (Sigma last)
+  double *xd, *xvt, *xrmp, *xrmm, *xe;
   int *nshift;
   fg_dev_tables t;
   double dt, omega_frame;
@@ -675,8 +810,8 @@ __device__ __forceinline__ float lane_next(float x) {
 // QuantitiesAdvection (TransportEuler.cu:229-245) on a lane's CL cells with
 // velocity V (one value per cell, or the ring's uniform residual when
 // UNIFORM); the reference's operations in its order
-template <int CL, bool UNIFORM>
-__device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&Vc)[CL], double Vu,
+template <int CL, bool UNIFORM, int NQT>
+__device__ __forceinline__ void wave_sweep(double (&q)[NQT][CL], const double (&Vc)[CL], double Vu,
                                            const fg_dev_tables &t, int i, double dt) {
   const double invdxtheta = t.invdxtheta[i], dxtheta = t.dxtheta[i];
   const double dxrad = (t.Rsup[i] - t.Rinf[i]) * dt;
@@ -685,7 +820,7 @@ __device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&V
   double dens[CL], DS[CL];
   // DensStar from Sigma (ComputeStarTheta(Dens), TransportEuler.cu:232)
 #pragma unroll
-  for (int c = 0; c < CL; c++) dens[c] = q[NQ - 1][c];
+  for (int c = 0; c < CL; c++) dens[c] = q[NQT - 1][c];
   {
     const double Wm = lane_prev(dens[CL - 1]), Wp = lane_next(dens[0]);
     double D[CL];
@@ -703,7 +838,7 @@ __device__ __forceinline__ void wave_sweep(double (&q)[NQ][CL], const double (&V
 #pragma unroll
   for (int c = 0; c < CL; c++) rd[c] = recip64(dens[c] + 1e-20).r;
 #pragma unroll
-  for (int qq = 0; qq < NQ; qq++) {
+  for (int qq = 0; qq < NQT; qq++) {
     // one quantity at a time: keeps the live registers to this quantity's
     // stencil (the scheduler would otherwise interleave all five)
     __builtin_amdgcn_sched_barrier(0);
@@ -779,12 +914,15 @@ __device__ __forceinline__ void wave_sweep32(double (&q)[NQ][CL], const float (&
 #undef V
 }
 
-// chunk `chunk` of ring i, one wave; F32 selects the FP32 bulk-mode sweeps
-template <int CL, bool F32>
+// chunk `chunk` of ring i, one wave; F32 selects the FP32 bulk-mode sweeps;
+// NQT = 6 adds the energy of an adiabatic deck (FP64 only)
+template <int CL, bool F32, int NQT = 5>
 __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   constexpr int N = 64 * CL, P = N - TAZW_HL - TAZW_HR;
-  // LDS: the load stage, then the four output stages (row 0 reused)
-  __shared__ double ostage[4][N];
+  constexpr int NOUT = NQT == 6 ? 5 : 4;
+  static_assert(NQT == 5 || !F32, "the FP32 sweeps carry five quantities");
+  // LDS: the load stage, then the output stages (row 0 reused)
+  __shared__ double ostage[NOUT][N];
   double *stage = ostage[0];
   const int ns = a.ns, l = threadIdx.x;
   const int g0 = chunk * P - TAZW_HL;  // unwrapped index of the chunk's first cell
@@ -793,7 +931,7 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   // loads: coalesced global reads into the chunk's LDS stage, then CL
   // consecutive cells per lane (a one-wave workgroup: the barriers only
   // order the LDS accesses of the wave)
-  double q[NQ][CL];
+  double q[NQT][CL];
   using vel_t = typename std::conditional<F32, float, double>::type;
   vel_t vres[CL];
   const double VMed = a.vmed[i];
@@ -811,22 +949,22 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   }
   // every field's loads are issued before the first is staged, so the six
   // fields cost one memory latency, not six
-  double raw[NQ + 1][CL];
+  double raw[NQT + 1][CL];
 #pragma unroll
-  for (int qq = 0; qq <= NQ; qq++) {
-    const double *src = qq < NQ ? a.q[qq] : a.vt;
+  for (int qq = 0; qq <= NQT; qq++) {
+    const double *src = qq < NQT ? a.q[qq] : a.vt;
 #pragma unroll
     for (int m = 0; m < CL; m++) raw[qq][m] = src[base + jl[m]];
   }
 #pragma unroll
-  for (int qq = 0; qq <= NQ; qq++) {
+  for (int qq = 0; qq <= NQT; qq++) {
 #pragma unroll
     for (int m = 0; m < CL; m++) stage[l + 64 * m] = raw[qq][m];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < CL; c++) {
       const double v = stage[l * CL + c];
-      if (qq < NQ)
+      if (qq < NQT)
         q[qq][c] = v;
       else
         vt0[c] = v - VMed;  // ComputeResidualsKernel
@@ -844,12 +982,12 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   for (int c = 0; c < CL; c++) vres[c] = (vel_t)(a.fast ? vt0[c] : vt0[c] + vuni);
   if (!a.fast) vuni = 0.0;
   if (chunk == 0 && l == 0) a.nshift[i] = nsh;
-  if constexpr (F32) {
+  if constexpr (F32 && NQT == 5) {
     wave_sweep32<CL, false>(q, vres, 0.0f, t, i, (float)a.dt);
     wave_sweep32<CL, true>(q, vres, (float)vuni, t, i, (float)a.dt);
   } else {
-    wave_sweep<CL, false>(q, vres, 0.0, t, i, a.dt);
-    wave_sweep<CL, true>(q, vres, vuni, t, i, a.dt);
+    wave_sweep<CL, false, NQT>(q, vres, 0.0, t, i, a.dt);
+    wave_sweep<CL, true, NQT>(q, vres, vuni, t, i, a.dt);
   }
   // vtheta from the angular momenta (Kernels.cu:1206-1207): cells j - 1
   // and j of the unshifted ring
@@ -857,11 +995,11 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   const recip64 rRm(Rm);
   double vt[CL];
   {
-    const double Wm = lane_prev(q[2][CL - 1]), Sm = lane_prev(q[NQ - 1][CL - 1]);
+    const double Wm = lane_prev(q[2][CL - 1]), Sm = lane_prev(q[NQT - 1][CL - 1]);
 #pragma unroll
     for (int c = 0; c < CL; c++) {
-      const double s = q[NQ - 1][c];
-      vt[c] = rRm.div(((c ? q[2][c - 1] : Wm) + q[3][c]) / (s + (c ? q[NQ - 1][c - 1] : Sm) + 1e-15)) -
+      const double s = q[NQT - 1][c];
+      vt[c] = rRm.div(((c ? q[2][c - 1] : Wm) + q[3][c]) / (s + (c ? q[NQT - 1][c - 1] : Sm) + 1e-15)) -
               Rm * a.omega_frame;
     }
   }
@@ -874,10 +1012,11 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   // back (one LDS round trip instead of four)
 #pragma unroll
   for (int c = 0; c < CL; c++) {
-    ostage[0][l * CL + c] = q[NQ - 1][c];
+    ostage[0][l * CL + c] = q[NQT - 1][c];
     ostage[1][l * CL + c] = vt[c];
     ostage[2][l * CL + c] = q[0][c];
     ostage[3][l * CL + c] = q[1][c];
+    if constexpr (NQT == 6) ostage[4][l * CL + c] = q[4][c];
   }
   __syncthreads();
 #pragma unroll
@@ -892,14 +1031,15 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
       a.xvt[lo] = ostage[1][k];
       a.xrmp[lo] = ostage[2][k];
       a.xrmm[lo] = ostage[3][k];
+      if constexpr (NQT == 6) a.xe[lo] = ostage[4][k];
     }
   }
 }
 
 // blockIdx.x = chunk, blockIdx.y = ring
-template <int CL, bool F32>
+template <int CL, bool F32, int NQT = 5>
 __global__ __launch_bounds__(64) void k_tazw(taz_args a) {
-  tazw_body<CL, F32>(a, blockIdx.y, blockIdx.x);
+  tazw_body<CL, F32, NQT>(a, blockIdx.y, blockIdx.x);
 }
 
 // cells per lane of the wave kernel for a ring of ns cells: CL = 4 or 5
@@ -932,6 +1072,14 @@ static int tazw_cells(int ns, int *nchunks) {
 struct fin_args {
   const double *xd, *xvt, *xrmp, *xrmm, *press;
   double *dens, *vr, *vt, *temp;
+  // adiabatic decks: the transported energy in, Energy out; the sound speed
+  // field of the CFL pass (the one of the DT's start, SourceEuler.cu:208),
+  // recomputed from the final state when this substep closes the DT
+  const double *xe;
+  double *energy, *cs;
+  double gamma;
+  int adiabatic, cs_refresh;
+  const double *sgacc;  // self-gravity: ring means of g_r for vtheta0 (else NULL)
   double *vmoy, *newdt;
   double *fpart;  // FP32 mode: per-ring compensated star-force partials (4 per ring), else NULL
   int *crash;     // set to 1 when a stored density is negative or not finite
@@ -1028,7 +1176,7 @@ __device__ __forceinline__ void ring_min_store(double best, double *red, double 
 }
 
 #define FIN_OV_K 16  // cells per thread of the overlapped CFL pass (rings up to 3072 cells)
-template <int FIN_B>
+template <int FIN_B, bool ADI = false>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[4];
   __shared__ double s_vmoy;
@@ -1038,8 +1186,9 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   double lam_i = 0, vr0_i = 0, vt0_i = 0, lam_n = 0, vr0_n = 0, vt0_n = 0;
   const bool damp_i = a.evanescent && t.ev_zone[i];
   const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
-  if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i);
-  if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n);
+  if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i, a.sgacc);
+  if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n, a.sgacc);
+  constexpr bool adi = ADI;  // an adiabatic deck (a.adiabatic): energy and per-cell sound speed
   double fsx = 0.0, fcx = 0.0, fsy = 0.0, fcy = 0.0;
   const double Rm = t.Rmed[i], surf = t.Surf[i];
   // star at the origin (x = y = 0, no smoothing): the cell's distance is
@@ -1064,7 +1213,10 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         }
         ld[u] = a.xd[l];
         lvt[u] = a.xvt[l];
-        if (a.temp) lp[u] = a.press[l];
+        if (adi)
+          lp[u] = a.xe[l];
+        else if (a.temp)
+          lp[u] = a.press[l];
       }
     }
 #pragma unroll
@@ -1084,11 +1236,21 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
-    if (a.temp) a.temp[l] = lp[u] / s;
+    if (adi) {
+      double e = lp[u];
+      if (damp_i) e = damp_toward(e, lam_i, t.EnergyMed[i]);
+      a.energy[l] = e;
+      bad |= !(e >= 0.0 && e < HUGE_VAL);
+      if (a.temp) a.temp[l] = (a.gamma - 1.0) * e / s;              // Kernels.cu:211
+      if (a.cs_refresh) a.cs[l] = sqrt(a.gamma * (a.gamma - 1.0) * e / s);  // Kernels.cu:181
+    } else if (a.temp) {
+      a.temp[l] = lp[u] / s;
+    }
     if (i == nr - 1) {  // ghost ring after transport: all zero
       a.dens[l + ns] = 0.0;
       a.vr[l + ns] = 0.0;
       a.vt[l + ns] = 0.0;
+      if (adi) a.energy[l + ns] = 0.0;
     }
   }
   }
@@ -1155,18 +1317,19 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       if (a.fpart) star_force();
     }
     if (tid >= 64 && i > 0) {
-      const double invdt1 = cs / mindx;
+      const double invdt1r = cs / mindx;
       const recip64 rdr(dxrad), rdt(dxtheta);
 #pragma unroll
       for (int u0 = 0; u0 < FIN_OV_K; u0 += 4) {
-        double pvr[4], prp[4], prm[4], pd0[4], pd1[4];
+        double pvr[4], prp[4], prm[4], pd0[4], pd1[4], pcs[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int j = tid - 64 + (u0 + u) * 192;
-          pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = 0.0;
+          pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = pcs[u] = 0.0;
           if (j < ns) {
             const long l = base + j;
             pvr[u] = a.vr[l];
+            if (adi) pcs[u] = a.cs[l];
             if (radial_n) {
               prp[u] = a.xrmp[l];
               prm[u] = a.xrmm[l + ns];
@@ -1183,6 +1346,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
           const double vr = pvr[u];
           double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
           if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+          const double invdt1 = adi ? pcs[u] / mindx : invdt1r;
           const double invdt2 = rdr.div(fabs(vr));
           double dvr = vrn - vr;
           double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
@@ -1229,14 +1393,15 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const double vm = s_vmoy;
   const recip64 rdr(dxrad), rdt(dxtheta);
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
-    double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B];
+    double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B], pcs[FIN_B];
 #pragma unroll
     for (int u = 0; u < FIN_B; u++) {
       const int j = j0 + u * 256;
-      pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = 0.0;
+      pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = pcs[u] = 0.0;
       if (j < ns) {
         const long l = base + j;
         pvr[u] = a.vr[l];  // written above by this thread
+        if (adi) pcs[u] = a.cs[l];
         if (radial_n) {
           prp[u] = a.xrmp[l];
           prm[u] = a.xrmm[l + ns];
@@ -1253,7 +1418,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
     if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
     double vres = a.fast ? (ring[j] - vm) : ring[j];
-    double invdt1 = cs / mindx;
+    double invdt1 = (adi ? pcs[u] : cs) / mindx;
     double invdt2 = rdr.div(fabs(vr));
     double invdt3 = rdt.div(fabs(vres));
     double dvr = vrn - vr;
@@ -1274,12 +1439,13 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
 // contiguous band of rings, so neighbouring rings, which read each other's
 // momenta and densities, are processed side by side under one L2.
 #define FG_NXCD 8
+template <bool ADI>
 __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   extern __shared__ double lds_fin[];
   const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
   const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
   if (i >= a.nr) return;
-  fin_ring<4>(a, i, lds_fin);
+  fin_ring<4, ADI>(a, i, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -1820,14 +1986,18 @@ extern "C" int fg_profile_read(fg_ctx *c, double *ms, long *calls, int max) {
 // The fused path's buffers are published together, only once every
 // allocation succeeded (a failed attempt frees what it got and a later call
 // starts again).
+// arrays in the fused pool: B (3), R (5), X (4), pressure, spare; adiabatic
+// or self-gravity decks add EnergyInt, two energies and the two kicks
+static int fused_pool_arrays(const fg_ctx *c) { return (c->p.adiabatic || c->p.self_gravity) ? 19 : 14; }
+
 static fg_status fused_alloc(fg_ctx *c) {
   if (c->fu_ring) return FG_OK;
   const size_t S = (size_t)c->S;
   const size_t fpart_bytes = 4 * (size_t)(c->nr + 1) * sizeof(double);
-  // 14 arrays: B (3), R (5), X (4), pressure, spare
+  const size_t pool_bytes = (size_t)fused_pool_arrays(c) * S * sizeof(double);
   double *pool = nullptr, *fpart = nullptr, *h = nullptr, *d = nullptr;
-  hipError_t e = hipMalloc((void **)&pool, 14 * S * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(pool, 0, 14 * S * sizeof(double));
+  hipError_t e = hipMalloc((void **)&pool, pool_bytes);
+  if (e == hipSuccess) e = hipMemset(pool, 0, pool_bytes);
   if (e == hipSuccess) e = hipMalloc((void **)&fpart, fpart_bytes);
   if (e == hipSuccess) e = hipMemset(fpart, 0, fpart_bytes);
   if (e == hipSuccess)
@@ -1867,13 +2037,27 @@ static fused_bufs bufs_of(fg_ctx *c) {
   b.Xrmp = p + 10 * S;
   b.Xrmm = p + 11 * S;
   b.P = p + 12 * S;
+  const bool more = fused_pool_arrays(c) > 14;
+  b.Beint = more ? p + 14 * S : nullptr;
+  b.Re = more ? p + 15 * S : nullptr;
+  b.Xe = more ? p + 16 * S : nullptr;
+  b.kick_r = more ? p + 17 * S : nullptr;
+  b.kick_t = more ? p + 18 * S : nullptr;
   return b;
 }
 
 static fg_status fused_supported(fg_ctx *c) {
   const fg_params &p = c->p;
-  if (p.adiabatic || p.self_gravity || p.open_inner || p.non_reflecting || !p.evanescent) {
-    fg_set_error("fused path: deck needs the per-routine path (adiabatic, self-gravity or non-evanescent edges)");
+  if (p.open_inner || p.non_reflecting || !p.evanescent) {
+    fg_set_error("fused path: deck needs the per-routine path (open or non-reflecting edges)");
+    return FG_E_UNSUPPORTED;
+  }
+  if ((p.adiabatic || p.self_gravity) && p.precision == FG_PREC_FP32) {
+    fg_set_error("fused path: the FP32 bulk mode takes locally isothermal decks without self-gravity");
+    return FG_E_UNSUPPORTED;
+  }
+  if (p.adiabatic && c->nr < 3) {
+    fg_set_error("fused path: an adiabatic deck needs NRAD >= 3");
     return FG_E_UNSUPPORTED;
   }
   if (c->ns > 4096) {
@@ -1963,7 +2147,8 @@ extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
   if (fg_status s = fused_alloc(c)) return s;
   size_t shm = (size_t)(c->ns + 256) * sizeof(double);
   hipLaunchKernelGGL(k_cfl_ring, dim3(c->nr), dim3(256), shm, c->stream, c->Vmoy, c->newDT, c->f[FG_VRAD],
-                     c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->f[FG_DENS], (const double *)nullptr,
+                     c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->f[FG_DENS],
+                     c->p.adiabatic ? (const double *)c->f[FG_ENERGY] : (const double *)nullptr,
                      (int *)(c->d_fused + FG_FUSED_CRASH_SLOT), c->t, c->nr, c->ns, c->p.fast_transport);
   FG_LAUNCH_CHECK();
   return fused_reductions(c, DeltaT);
@@ -1977,7 +2162,7 @@ fg_status fg_fused_check(fg_ctx *c, double cfl, const double *force2) {
   volatile int *flag = (volatile int *)(c->h_fused + FG_FUSED_CRASH_SLOT);
   if (*flag) {
     *flag = 0;
-    fg_set_error("crash: negative or non-finite density");
+    fg_set_error("crash: negative or non-finite density or energy");
     return FG_E_CRASH;
   }
   return fg_check_substep_results(cfl, force2);
@@ -2005,7 +2190,7 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
                                    const fg_planet_pot *planets, int nb, double ix, double iy,
-                                   double DeltaT_next, int want_temperature) {
+                                   double DeltaT_next, int flags) {
   if (!c) return FG_E_INVALID;
   if (fg_status s = fused_supported(c)) return s;
   if (fg_status s = fused_alloc(c)) return s;
@@ -2013,8 +2198,15 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     fg_set_error("fg_step_fused: planet count %d", nb);
     return FG_E_INVALID;
   }
+  const int want_temperature = flags & FG_STEP_TEMPERATURE;
   const int nr = c->nr, ns = c->ns;
+  const bool adi = c->p.adiabatic, sg = c->p.self_gravity;
   fused_bufs b = bufs_of(c);
+  // this substep's viscosity: refreshed at the close of the last DT, it
+  // replaces nu after the first damping pass
+  const double *nu = c->nu_pending ? c->nu_next : c->nu;
+  if (sg)
+    if (fg_status s = fg_sg_fused(c, dt, b.kick_r, b.kick_t)) return s;
   // k_src
   src_args sa;
   sa.dens = c->f[FG_DENS];
@@ -2023,8 +2215,16 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   sa.odens = b.Bd;
   sa.ovr = b.Bvr;
   sa.ovt = b.Bvt;
-  sa.opress = want_temperature ? b.P : nullptr;
-  sa.nu = c->nu;
+  sa.opress = (want_temperature && !adi) ? b.P : nullptr;
+  sa.nu = nu;
+  sa.nu1 = c->nu;
+  sa.energy = adi ? c->f[FG_ENERGY] : nullptr;
+  sa.oeint = b.Beint;
+  sa.gamma = c->p.adiabatic_index;
+  sa.kick_r = b.kick_r;
+  sa.kick_t = b.kick_t;
+  sa.sg1 = sg ? c->sgacc_prev : nullptr;
+  sa.sg2 = sg ? c->sgacc_mean : nullptr;
   sa.t = c->t;
   sa.P.nb = nb;
   for (int k = 0; k < nb; k++) {
@@ -2084,20 +2284,49 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     else
       hipLaunchKernelGGL(k_src32<16>, g32, dim3(SRC32_TPB), src32_lds_bytes(16), c->stream, s3);
   } else {
-    hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB>), dim3((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH), dim3(SRC64_TPB), 0, c->stream,
-                       sa);
+    const dim3 gs((ns + TW - 1) / TW, (nr + 1 + TH - 1) / TH);
+    const int mode = (adi ? 1 : 0) | (sg ? 2 : 0);
+    if (mode == 0)
+      hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB, 0>), gs, dim3(SRC64_TPB), 0, c->stream, sa);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB, 1>), gs, dim3(SRC64_TPB), 0, c->stream, sa);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB, 2>), gs, dim3(SRC64_TPB), 0, c->stream, sa);
+    else
+      hipLaunchKernelGGL((k_src<TH, TW, SRC64_TPB, 3>), gs, dim3(SRC64_TPB), 0, c->stream, sa);
   }
   FG_LAUNCH_CHECK();
+  if (adi) {  // Substep3 (counted with k_src by fg_profile)
+    src3_args s3;
+    s3.dens = b.Bd;
+    s3.vr = b.Bvr;
+    s3.vt = b.Bvt;
+    s3.eint = b.Beint;
+    s3.energy = c->f[FG_ENERGY];
+    s3.nu = nu;
+    s3.t = c->t;
+    s3.dt = dt;
+    s3.gamma = c->p.adiabatic_index;
+    s3.lr01 = c->lr01;
+    s3.lr12 = c->lr12;
+    s3.nr = nr;
+    s3.ns = ns;
+    s3.cooling = c->p.cooling;
+    hipLaunchKernelGGL(k_src3, dim3((ns + 63) / 64, (nr + 3) / 4), dim3(256), 0, c->stream, s3);
+    FG_LAUNCH_CHECK();
+  }
   // k_trad
   trad_args ta;
   ta.dens = b.Bd;
   ta.vr = b.Bvr;
   ta.vt = b.Bvt;
+  ta.energy = adi ? c->f[FG_ENERGY] : nullptr;
   ta.out[0] = b.Rrmp;
   ta.out[1] = b.Rrmm;
   ta.out[2] = b.Rtmp;
   ta.out[3] = b.Rtmm;
-  ta.out[4] = b.Rd;
+  ta.out[4] = adi ? b.Re : b.Rd;
+  ta.out[5] = b.Rd;
   ta.t = c->t;
   ta.dt = dt;
   ta.omega_frame = omega_frame;
@@ -2110,8 +2339,10 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   const dim3 gtrad(nr + ta.nstrips * ((nr + ta.band - 1) / ta.band));
   if (fp32)
     hipLaunchKernelGGL(k_trad32, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
+  else if (adi)
+    hipLaunchKernelGGL(k_trad<6>, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
   else
-    hipLaunchKernelGGL(k_trad, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
+    hipLaunchKernelGGL(k_trad<5>, gtrad, dim3(TRAD_TPB), 0, c->stream, ta);
   FG_LAUNCH_CHECK();
   // k_taz
   taz_args za;
@@ -2121,11 +2352,13 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.q[1] = b.Rrmm;
   za.q[2] = b.Rtmp;
   za.q[3] = b.Rtmm;
-  za.q[4] = b.Rd;
+  za.q[4] = adi ? b.Re : b.Rd;
+  za.q[5] = b.Rd;
   za.xd = b.Xd;
   za.xvt = b.Xvt;
   za.xrmp = b.Xrmp;
   za.xrmm = b.Xrmm;
+  za.xe = b.Xe;
   za.nshift = c->Nshift;
   za.t = c->t;
   za.dt = dt;
@@ -2134,7 +2367,11 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.ns = ns;
   za.fast = c->p.fast_transport;
   prof_mark(c);
-  if (!fp32) {
+  if (adi) {
+    // six quantities: CL = 4 keeps two waves per SIMD
+    const dim3 gw((ns + TAZW_P32 - 1) / TAZW_P32, nr);
+    hipLaunchKernelGGL((k_tazw<4, false, 6>), gw, dim3(64), 0, c->stream, za);
+  } else if (!fp32) {
     int nch = 0;
     const int cl = tazw_cells(ns, &nch);
     const dim3 gw(nch, nr);
@@ -2172,12 +2409,35 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.fast = c->p.fast_transport;
   fa.fpart = fp32 ? c->fu_fpart : nullptr;
   fa.crash = (int *)(c->d_fused + FG_FUSED_CRASH_SLOT);
+  fa.nu = nu;
+  fa.xe = b.Xe;
+  fa.energy = adi ? c->f[FG_ENERGY] : nullptr;
+  fa.cs = c->f[FG_SOUNDSPEED];
+  fa.gamma = c->p.adiabatic_index;
+  fa.adiabatic = adi ? 1 : 0;
+  fa.cs_refresh = (adi && (flags & FG_STEP_CLOSES_DT)) ? 1 : 0;
+  fa.sgacc = sg ? c->sgacc_mean : nullptr;
   prof_mark(c);
-  hipLaunchKernelGGL(k_fin, dim3(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD)), dim3(256), (size_t)ns * sizeof(double),
-                     c->stream, fa);
+  const dim3 gfin(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD));
+  if (adi)
+    hipLaunchKernelGGL(k_fin<true>, gfin, dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
+  else
+    hipLaunchKernelGGL(k_fin<false>, gfin, dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   FG_LAUNCH_CHECK();
   prof_mark(c);
-  return fused_reductions(c, DeltaT_next, fp32);
+  if (c->nu_pending) {  // the refreshed viscosity is now the current one
+    std::swap(c->nu, c->nu_next);
+    c->nu_pending = 0;
+  }
+  if (fg_status s = fused_reductions(c, DeltaT_next, fp32)) return s;
+  if (fa.cs_refresh && c->p.viscosity_alpha) {
+    // the next DT's alpha viscosity from the ring means of the new sound
+    // speed (FViscosity via Make1Dprofile, as fg_ring_mean_cs computes it);
+    // the next substep's first damping pass still uses this one
+    if (fg_status s = fg_viscosity_into(c, c->nu_next)) return s;
+    c->nu_pending = 1;
+  }
+  return FG_OK;
 }
 
 // ================================================== batched ensemble ====
@@ -2554,6 +2814,11 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fa.fast = c->p.fast_transport;
   fa.fpart = c->fu_fpart;
   fa.crash = (int *)(c->d_fused + FG_FUSED_CRASH_SLOT);
+  fa.xe = nullptr;
+  fa.energy = fa.cs = nullptr;
+  fa.gamma = 0.0;
+  fa.adiabatic = fa.cs_refresh = 0;
+  fa.sgacc = nullptr;
   d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
   d.ncell = (long)nr * ns;
   d.B = red_blocks(d.ncell);

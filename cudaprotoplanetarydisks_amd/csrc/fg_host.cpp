@@ -980,12 +980,14 @@ extern "C" fg_status fg_sim_create(const char *parfile, const fg_sim_options *op
   s->PhysicalTimeInitial = s->PhysicalTime;
   s->prof.time = s->PhysicalTime;
   s->prof.time0 = s->PhysicalTimeInitial;
-  // the fused path covers locally isothermal decks with evanescent edges
-  // and a per-substep CFL; anything else stays on the per-routine path
-  const bool fusable = !d.Adiabatic && !d.SelfGravity && d.Evanescent && !d.OpenInner && !d.NonReflecting &&
-                       !d.SloppyCFL && d.IsDisk && ns <= 4096;
-  // the FP32 bulk mode exists only as fused kernels
-  if (s->opt.precision == FG_PREC_FP32 && !fusable) {
+  // the fused path covers decks with evanescent edges and a per-substep
+  // CFL, isothermal or adiabatic, with or without self-gravity; anything
+  // else stays on the per-routine path
+  const bool fusable = d.Evanescent && !d.OpenInner && !d.NonReflecting && !d.SloppyCFL && d.IsDisk && ns <= 4096 &&
+                       (!d.Adiabatic || nr >= 3);
+  // the FP32 bulk mode exists only as fused kernels, isothermal without
+  // self-gravity
+  if (s->opt.precision == FG_PREC_FP32 && (!fusable || d.Adiabatic || d.SelfGravity)) {
     host_error("FP32 bulk mode needs a locally isothermal deck with evanescent edges, per-substep CFL and "
                "NSEC <= 4096");
     fg_sim_destroy(s);
@@ -1092,7 +1094,7 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
   fused_launch L;
   CHECK(fused_plan(s, cfl, force, L));
   CHECK(fg_step_fused(c, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY, L.DeltaT_next,
-                      d.Write_Temperature));
+                      (d.Write_Temperature ? FG_STEP_TEMPERATURE : 0) | (L.done ? FG_STEP_CLOSES_DT : 0)));
   CHECK(substep_diagnostics(s));
   fused_close(s, L, closed);
   return FG_OK;

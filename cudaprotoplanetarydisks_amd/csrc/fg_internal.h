@@ -22,6 +22,10 @@ void fg_set_error(const char *fmt, ...);
 // once per device, with that device current: locate the device's copy of
 // the scratch array of fg_disk_force_hill_noalloc (fg_reduce.hip)
 fg_status fg_noalloc_setup(int device);
+// self-gravity part of a fused substep (fg_sg.hip)
+fg_status fg_sg_fused(fg_ctx *c, double dt, double *kick_r, double *kick_t);
+// alpha viscosity of the current sound speed into nu_out (fg_steps.hip)
+fg_status fg_viscosity_into(fg_ctx *c, double *nu_out);
 
 // Result block of the fused path (fg_ctx::h_fused, coherent pinned host
 // memory the final kernels write straight into): [0..1] star force,
@@ -109,6 +113,12 @@ struct fg_ctx {
   double *DRR, *DPP, *DRP, *Work, *QRStar, *DensStar, *dq, *TempShift, *Vradial, *Vazimutal;
   // ring arrays (nr+1)
   double *Vmoy, *newDT, *DT1D, *VMed, *nu, *cs_mean, *sgacc_mean;
+  // fused path: the previous self-gravity evaluation's ring means (the
+  // first damping pass of a substep uses them), and the viscosity of an
+  // adiabatic alpha-viscosity deck refreshed at the close of a DT, which
+  // takes over from nu after the next substep's first damping pass
+  double *sgacc_prev, *nu_next;
+  int nu_pending;
   int *Nshift, *NoSplit;
   double *LostByDisk;  // ns
   // reduction scratch

@@ -19,6 +19,9 @@
 // i.e. the same products and sums, in the same order, as the recursive
 // FFT of the host oracle.  The kernel spectra are constant and are
 // transformed once (the reference redoes this every call).
+#include <utility>
+
+#include "fg_cell.h"
 #include "fg_internal.h"
 
 extern "C" void fg_fft_twiddles(int L, int sign, double *cs);  // fg_hostmath.cpp
@@ -218,14 +221,24 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
   }
 }
 
-// ComputeFFTKernel, Kernels.cu:840-868 (density half of the padded grids)
-__global__ void k_sg_fill(double2 *Sr, double2 *St, const double *dens, fg_dev_tables t, int nr, int ns) {
+// Sigma of cell l after the substep's first damping pass (the density part
+// of EvanescentBoundaryKernel, Kernels.cu:978-1004; the fused path applies
+// that pass in its source kernel, after the self-gravity field it needs)
+__device__ __forceinline__ double sigma_damped(const fg_dev_tables &t, int i, double dt, int damp, double s) {
+  if (damp && t.ev_zone[i]) s = damp_toward(s, t.ev_coef[i] * dt / t.ev_period[i], t.SigmaMed[i]);
+  return s;
+}
+
+// ComputeFFTKernel, Kernels.cu:840-868 (density half of the padded grids);
+// damp: Sigma as after the first damping pass of a substep of length dt
+__global__ void k_sg_fill(double2 *Sr, double2 *St, const double *dens, fg_dev_tables t, double dt, int damp,
+                          int nr, int ns) {
   const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= 2L * nr * ns) return;
   const int i = (int)(l / ns);
   double sr = 0.0, st = 0.0;
   if (i < nr) {
-    const double var = dens[l] * t.sg_sqrt_ratio[i];
+    const double var = sigma_damped(t, i, dt, damp, dens[l]) * t.sg_sqrt_ratio[i];
     sr = var;
     st = var * t.Rmed[i] / t.Rmed[0];
   }
@@ -260,6 +273,32 @@ __global__ void k_sg_acc(double *accr, double *acct, const double *dens, const d
   double a = Gr[l].x * t.sg_norm_r[i];
   a += 1.0 * dens[l] * rstep * tstep / eps;
   accr[l] = a;
+}
+
+// The fused path's ComputeSgAccKernel: the accelerations from the density
+// after the first damping pass, and the kick that Update_sgvelocityKernel
+// (Kernels.cu:915-932) adds to VradInt / VthetaInt, for the source kernel
+// to add (0 on ring 0, which the kick leaves alone).  The neighbours'
+// accelerations are recomputed from the same expressions.
+__global__ void k_sg_acc_fused(double *accr, double *acct, double *kick_r, double *kick_t, const double *dens,
+                               const double2 *Gr, const double2 *Gt, fg_dev_tables t, double rstep, double tstep,
+                               double eps, double dt, int damp, int nr, int ns) {
+  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= (long)nr * ns) return;
+  const int i = (int)(l / ns), j = (int)(l - (long)i * ns);
+  auto acc_r = [&](long m, int ii) {
+    double a = Gr[m].x * t.sg_norm_r[ii];
+    a += 1.0 * sigma_damped(t, ii, dt, damp, dens[m]) * rstep * tstep / eps;
+    return a;
+  };
+  const long lm = (long)i * ns + (j == 0 ? ns - 1 : j - 1);
+  const double ar = acc_r(l, i), at = Gt[l].x * t.sg_norm_t[i], atm = Gt[lm].x * t.sg_norm_t[i];
+  accr[l] = ar;
+  acct[l] = at;
+  kick_r[l] = i > 0 ? dt * ((t.Rinf[i] - t.Rmed[i - 1]) * ar + (t.Rmed[i] - t.Rinf[i]) * acc_r(l - ns, i - 1)) *
+                          t.invdiffRmed[i]
+                    : 0.0;
+  kick_t[l] = 0.5 * dt * (at + atm);
 }
 
 // Make1Dprofile of SG_Accr: sequential sum over the sectors of each ring
@@ -433,7 +472,7 @@ extern "C" fg_status fg_selfgravity(fg_ctx *c, double dt, int update) {
   const int nr = c->nr, ns = c->ns;
   const long n = 2L * nr * ns, z = (long)nr * ns;
   hipLaunchKernelGGL(k_sg_fill, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Sr,
-                     (double2 *)c->sg_St, c->f[FG_DENS], c->t, nr, ns);
+                     (double2 *)c->sg_St, c->f[FG_DENS], c->t, 0.0, 0, nr, ns);
   FG_LAUNCH_CHECK();
   if (fg_status s = fft2d(c, c->sg_St, c->sg_Sr, 0)) return s;
   hipLaunchKernelGGL(k_sg_convolve, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Gr,
@@ -456,3 +495,31 @@ extern "C" fg_status fg_selfgravity(fg_ctx *c, double dt, int update) {
 }
 
 fg_status fg_selfgravity_kick(fg_ctx *c, double dt) { return fg_selfgravity(c, dt, 1); }
+
+// The self-gravity part of a fused substep, run before its source kernel:
+// the field of the density after the first damping pass, its ring means
+// (the previous evaluation's move to sgacc_prev, which that damping pass
+// still uses), and the kick the source kernel adds after Substep1.
+fg_status fg_sg_fused(fg_ctx *c, double dt, double *kick_r, double *kick_t) {
+  if (fg_status s = sg_prepare(c)) return s;
+  const int nr = c->nr, ns = c->ns;
+  const long n = 2L * nr * ns, z = (long)nr * ns;
+  const int damp = c->p.evanescent;
+  std::swap(c->sgacc_mean, c->sgacc_prev);
+  hipLaunchKernelGGL(k_sg_fill, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Sr,
+                     (double2 *)c->sg_St, c->f[FG_DENS], c->t, dt, damp, nr, ns);
+  FG_LAUNCH_CHECK();
+  if (fg_status s = fft2d(c, c->sg_St, c->sg_Sr, 0)) return s;
+  hipLaunchKernelGGL(k_sg_convolve, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Gr,
+                     (double2 *)c->sg_Gt, (const double2 *)c->sg_Krc, (const double2 *)c->sg_Ktc,
+                     (const double2 *)c->sg_Sr, (const double2 *)c->sg_St, n);
+  FG_LAUNCH_CHECK();
+  if (fg_status s = fft2d(c, c->sg_Gt, c->sg_Gr, 1)) return s;
+  hipLaunchKernelGGL(k_sg_acc_fused, dim3(fg_blocks(z, 256)), dim3(256), 0, c->stream, c->f[FG_SG_ACCR],
+                     c->f[FG_SG_ACCT], kick_r, kick_t, c->f[FG_DENS], (const double2 *)c->sg_Gr,
+                     (const double2 *)c->sg_Gt, c->t, c->p.sg_rstep, c->p.sg_tstep, c->p.sg_eps, dt, damp, nr, ns);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sg_ring_mean, dim3(nr), dim3(64), 0, c->stream, c->f[FG_SG_ACCR], c->sgacc_mean, ns);
+  FG_LAUNCH_CHECK();
+  return FG_OK;
+}

@@ -529,8 +529,8 @@ extern "C" fg_status fg_temperature(fg_ctx *c) {
   return FG_OK;
 }
 
-static fg_status refresh_viscosity(fg_ctx *c) {
-  hipLaunchKernelGGL(k_viscosity, dim3(fg_blocks(c->nr, 256)), dim3(256), 0, c->stream, c->nu, c->cs_mean,
+static fg_status refresh_viscosity(fg_ctx *c, double *nu_out) {
+  hipLaunchKernelGGL(k_viscosity, dim3(fg_blocks(c->nr, 256)), dim3(256), 0, c->stream, nu_out, c->cs_mean,
                      c->t.visc_pow15, c->t.visc_cav_on, c->t.visc_cav, c->nr, c->p.viscosity_alpha,
                      c->p.alphaviscosity, c->t.visc_value);
   FG_LAUNCH_CHECK();
@@ -539,10 +539,20 @@ static fg_status refresh_viscosity(fg_ctx *c) {
 
 extern "C" fg_status fg_ring_mean_cs(fg_ctx *c) {
   if (fg_status s = check(c)) return s;
+  c->nu_pending = 0;  // nu is current: a fused refresh held back is void
   hipLaunchKernelGGL(k_ring_mean, dim3(c->nr), dim3(256), c->ns * sizeof(double), c->stream, c->cs_mean,
                      F(c, FG_SOUNDSPEED), c->nr, c->ns);
   FG_LAUNCH_CHECK();
-  return refresh_viscosity(c);
+  return refresh_viscosity(c, c->nu);
+}
+
+// the alpha viscosity of the current sound speed field into nu_out (the
+// fused path's refresh at the close of a DT), leaving nu as it is
+fg_status fg_viscosity_into(fg_ctx *c, double *nu_out) {
+  hipLaunchKernelGGL(k_ring_mean, dim3(c->nr), dim3(256), c->ns * sizeof(double), c->stream, c->cs_mean,
+                     F(c, FG_SOUNDSPEED), c->nr, c->ns);
+  FG_LAUNCH_CHECK();
+  return refresh_viscosity(c, nu_out);
 }
 
 extern "C" fg_status fg_cfl(fg_ctx *c, double DeltaT, int *ncfl) {

@@ -218,14 +218,22 @@ fg_status fg_selfgravity(fg_ctx *ctx, double dt, int update);
 /* Everything AlgoGas does on the GPU for one substep after the planets
  * are advanced (SourceEuler.cu:255-295): CorrectVtheta (domega, 0 when
  * the frame does not co-rotate), the potential (FillForcesArrays), the
- * three boundary passes, pressure, Substep1/2, transport and, when
- * want_temperature, the temperature.  Also leaves the CFL sub-cycle count
- * for DeltaT_next (the next substep's ConditionCFL, SourceEuler.cu:228)
- * and the disk force on the star for it (SourceEuler.cu:241), fetched by
- * fg_fused_results.  Locally isothermal decks with evanescent edges. */
+ * three boundary passes, pressure, Substep1/2, the self-gravity kick
+ * (compute_selfgravity inside Substep1, SourceEuler.cu:322-330), Substep3
+ * of an adiabatic deck, transport and, with FG_STEP_TEMPERATURE in flags,
+ * the temperature.  Also leaves the CFL sub-cycle count for DeltaT_next
+ * (the next substep's ConditionCFL, SourceEuler.cu:228) and the disk force
+ * on the star for it (SourceEuler.cu:241), fetched by fg_fused_results.
+ * FG_STEP_CLOSES_DT marks the last substep of a DT: an adiabatic deck then
+ * recomputes the sound speed from the new state (AlgoGas does it when the
+ * next DT opens, SourceEuler.cu:207-213) for that CFL count and the next
+ * DT's alpha viscosity.  Decks with evanescent edges; isothermal or
+ * adiabatic, with or without self-gravity (FP32 bulk mode: isothermal
+ * without self-gravity). */
+enum { FG_STEP_TEMPERATURE = 1, FG_STEP_CLOSES_DT = 2 };
 fg_status fg_step_fused(fg_ctx *ctx, double dt, double omega_frame, double domega,
                         const fg_planet_pot *planets, int nb, double indirect_x, double indirect_y,
-                        double DeltaT_next, int want_temperature);
+                        double DeltaT_next, int flags);
 /* CFL count and star force of the current state (fused reductions) */
 fg_status fg_fused_prepare(fg_ctx *ctx, double DeltaT);
 fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_mass);

@@ -44,6 +44,26 @@ def test_bad_density_returns_crash(deck, opts, bad):
         assert np.isfinite(d2).all() and (d2[:-1] > 0).all()
 
 
+@pytest.mark.parametrize("fused", [False, True], ids=["per-routine", "fused"])
+def test_bad_energy_returns_crash(deck, fused):
+    path = deck(Nrad=24, Nsec=48, Adiabatic="YES", AdiabaticIndex=1.4)
+    with Simulation(path, fused=fused) as sim:
+        sim.substep()
+        d, vr, vt, e = sim.state(energy=True)
+        good = e.copy()
+        e[5, 7] = -1e-9
+        sim.set_state(d, vr, vt, e)
+        print(f"crash case: energy, fused={fused}, cell (5, 7) = -1e-9", flush=True)
+        with pytest.raises(FargoError) as err:
+            sim.substep()
+        assert err.value.status == FG_E_CRASH
+        sim.set_state(d, vr, vt, good)
+        for _ in range(3):
+            sim.substep()
+        _, _, _, e2 = sim.state(energy=True)
+        assert np.isfinite(e2).all() and (e2[:-1] > 0).all()
+
+
 def test_crash_in_a_batch_disk(tmp_path):
     params = [dict(nrad=24, nsec=72, q=1e-3, alpha=1e-3, h=0.05, slope=0.0, flaring=0.0),
               dict(nrad=32, nsec=96, q=2e-3, alpha=2e-4, h=0.07, slope=-0.5, flaring=0.1)]
