@@ -23,6 +23,7 @@
 
 #include "fg_cell.h"
 #include "fg_internal.h"
+#include "fg_reduce.h"
 
 extern "C" void fg_fft_twiddles(int L, int sign, double *cs);  // fg_hostmath.cpp
 
@@ -38,6 +39,28 @@ struct fg_fft_plan {
   double2 *tw[2];  // per-level twiddle tables; [0] forward (sign -1), [1] inverse (+1)
 };
 
+// Sigma of cell l after the substep's first damping pass (the density part
+// of EvanescentBoundaryKernel, Kernels.cu:978-1004; the fused path applies
+// that pass in its source kernel, after the self-gravity field it needs)
+__device__ __forceinline__ double sigma_damped(const fg_dev_tables &t, int i, double dt, int damp, double s) {
+  if (damp && t.ev_zone[i]) s = damp_toward(s, t.ev_coef[i] * dt / t.ev_period[i], t.SigmaMed[i]);
+  return s;
+}
+
+// What a pass reads and writes besides its lines:
+//   FFT_PLAIN  lines in, lines out (in place)
+//   FFT_FILL   rows of the forward transform: row i < NRAD is formed from
+//              Sigma as ComputeFFTKernel does (Kernels.cu:840-868; grid 0
+//              the azimuthal, grid 1 the radial one); the zero rows of the
+//              padding are never transformed
+//   FFT_CONV   columns of the forward transform: rows >= zrows read as 0
+//              (the padding), and the spectrum is multiplied by the
+//              kernel's and stored negated into out[]
+//              (ComputeConvolutionKernel, Kernels.cu:871-890)
+//   FFT_HEAD   columns of the inverse transform: only rows < zrows are
+//              stored (the rest of the padded grid is never read)
+enum { FFT_PLAIN = 0, FFT_FILL = 1, FFT_CONV = 2, FFT_HEAD = 3 };
+
 struct fft_args {
   int nst;
   int p[FG_FFT_MAXST], L[FG_FFT_MAXST], S[FG_FFT_MAXST], off[FG_FFT_MAXST], sh[FG_FFT_MAXST];
@@ -48,18 +71,53 @@ struct fft_args {
   long elem_stride;     // distance between consecutive elements of a line
   long line_step;       // distance between consecutive lines
   int lines_per_block;  // lines staged together in LDS
+  // FFT_FILL: Sigma and the substep's first damping pass
+  const double *dens;
+  fg_dev_tables t;
+  double dt;
+  int damp;
+  int zrows;               // FFT_CONV / FFT_HEAD: rows of the unpadded grid
+  const double2 *kern[2];  // FFT_CONV: kernel spectra of grids 0 and 1
+  double2 *out[2];         // FFT_CONV: products
 };
 
+// element o of line `line` of grid g (blockIdx.y), as the pass reads it
+template <int MODE>
+__device__ __forceinline__ double2 fft_load(const fft_args &a, int g, int line, int o) {
+  if (line >= a.nlines) return make_double2(0.0, 0.0);
+  if constexpr (MODE == FFT_FILL) {
+    const fg_dev_tables &t = a.t;
+    const double var = sigma_damped(t, line, a.dt, a.damp, a.dens[(long)line * a.n + o]) * t.sg_sqrt_ratio[line];
+    return make_double2(g == 0 ? var * t.Rmed[line] / t.Rmed[0] : var, 0.0);
+  } else {
+    if (MODE == FFT_CONV && o >= a.zrows) return make_double2(0.0, 0.0);
+    return a.data[g][line * a.line_step + o * a.elem_stride];
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void fft_store(const fft_args &a, int g, int line, int o, double2 v) {
+  if (line >= a.nlines) return;
+  const long idx = line * a.line_step + o * a.elem_stride;
+  if constexpr (MODE == FFT_CONV) {
+    const double2 k = a.kern[g][idx];
+    a.out[g][idx] = make_double2(-1.0 * (k.x * v.x - k.y * v.y), -1.0 * (k.x * v.y + k.y * v.x));
+  } else {
+    if (MODE == FFT_HEAD && o >= a.zrows) return;
+    a.data[g][idx] = v;
+  }
+}
+
+template <int MODE>
 __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
   extern __shared__ double2 buf[];
   const int CB = a.lines_per_block, N = a.n, tid = threadIdx.x;
-  const int line0 = blockIdx.x * CB;
-  double2 *data = a.data[blockIdx.y];
+  const int line0 = blockIdx.x * CB, g = blockIdx.y;
   const int E = N * CB;
   // stage: element o of line cb at buf[o*CB + cb]
   for (int e = tid; e < E; e += FFT_TPB) {
-    const int cb = e % CB, o = e / CB, line = line0 + cb;
-    buf[e] = (line < a.nlines) ? data[line * a.line_step + o * a.elem_stride] : make_double2(0.0, 0.0);
+    const int cb = e % CB, o = e / CB;
+    buf[e] = fft_load<MODE>(a, g, line0 + cb, o);
   }
   __syncthreads();
   for (int d = a.nst - 1; d >= 0; d--) {
@@ -95,8 +153,8 @@ __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
     __syncthreads();
   }
   for (int e = tid; e < E; e += FFT_TPB) {
-    const int cb = e % CB, o = e / CB, line = line0 + cb;
-    if (line < a.nlines) data[line * a.line_step + o * a.elem_stride] = buf[e];
+    const int cb = e % CB, o = e / CB;
+    fft_store<MODE>(a, g, line0 + cb, o, buf[e]);
   }
 }
 
@@ -106,36 +164,48 @@ __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
 #define FFT2_TPB 1024
 #define FFT2_U 4
 
+// one radix-2 output, r = 0 with w0 then r = 1 with t1, accumulated from
+// 0.0 as the generic p-term loop does it (same products, same order)
+__device__ __forceinline__ double2 bfly2(double2 x0, double2 x1, double2 w0, double2 t1) {
+  double re = 0.0, im = 0.0;
+  re += x0.x * w0.x - x0.y * w0.y;
+  im += x0.x * w0.y + x0.y * w0.x;
+  re += x1.x * t1.x - x1.y * t1.y;
+  im += x1.x * t1.y + x1.y * t1.x;
+  return make_double2(re, im);
+}
+
 // The same transform for lengths N = 2^a q with at most one factor other
 // than 2 (q = 1, 3, 5 or a prime done as one direct DFT): every level above
 // the deepest is radix 2, so each S_d is a power of two and an output's
-// indices are shifts and masks; k = kk mod m_d is kk or kk - m_d for the
-// radix-2 levels and 0 on the deepest level (m = 1).  CB lines are staged
-// together (element o of line cb at buf[o*CB + cb]); global loads and
-// stores run along the line when the line is contiguous (rows) and across
-// the CB lines otherwise (columns).  Outputs are accumulated exactly as in
-// k_fft_lines (same twiddles, same order), so both give the same bits.
-template <int CB>
+// indices are shifts and masks.  CB lines are staged together (element o
+// of line cb at buf[o*CB + cb]); global loads and stores run along the
+// line when the line is contiguous (rows) and across the CB lines
+// otherwise (columns).
+//
+// The radix-2 levels run two at a time in registers.  With the upper level
+// of a pair at sub-length L and S sub-problems (the lower one at L/2 and
+// 2S), the four values A[s + j*S + 4S*k2], j = 0..3, of sub-problem s and
+// k2 < L/4 are all that the lower level needs for its outputs k2 and
+// k2 + L/4 of sub-problems s and s + S, and those four are all that the
+// upper level needs for its outputs k2 + i*L/4, i = 0..3, of sub-problem s.
+// One thread evaluates both levels of such a group: every output is the
+// same radix-2 expression of the same operands as when the levels run one
+// after the other through LDS, so both give the same bits (as does
+// k_fft_lines), with half the LDS round trips and barriers.
+template <int CB, int MODE>
 __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
   extern __shared__ double2 buf[];
   constexpr int LCB = CB == 1 ? 0 : CB == 2 ? 1 : CB == 4 ? 2 : 3;
   const int N = a.n, tid = threadIdx.x;
-  const int line0 = blockIdx.x * CB;
-  double2 *data = a.data[blockIdx.y];
+  const int line0 = blockIdx.x * CB, g = blockIdx.y;
   const int E = N * CB;
   if (a.elem_stride == 1) {
 #pragma unroll
-    for (int cb = 0; cb < CB; cb++) {
-      const int line = line0 + cb;
-      const double2 *src = data + (long)line * a.line_step;
-      for (int o = tid; o < N; o += FFT2_TPB)
-        buf[o * CB + cb] = (line < a.nlines) ? src[o] : make_double2(0.0, 0.0);
-    }
+    for (int cb = 0; cb < CB; cb++)
+      for (int o = tid; o < N; o += FFT2_TPB) buf[o * CB + cb] = fft_load<MODE>(a, g, line0 + cb, o);
   } else {
-    for (int e = tid; e < E; e += FFT2_TPB) {
-      const int cb = e & (CB - 1), o = e >> LCB, line = line0 + cb;
-      buf[e] = (line < a.nlines) ? data[line * a.line_step + o * a.elem_stride] : make_double2(0.0, 0.0);
-    }
+    for (int e = tid; e < E; e += FFT2_TPB) buf[e] = fft_load<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB);
   }
   __syncthreads();
   double2 out[FFT2_U];
@@ -172,8 +242,10 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
     }
     __syncthreads();
   }
-  // the radix-2 levels above it: r = 0 with w[0], r = 1 with w[kk] (kk < L)
-  for (int d = a.nst - 2; d >= 0; d--) {
+  int d = a.nst - 2;
+  if (d >= 0 && (d & 1) == 0) {
+    // an odd number of radix-2 levels: the lowest one alone (r = 0 with
+    // w[0], r = 1 with w[kk])
     const int L = a.L[d], sh = a.sh[d], m = L >> 1;
     const int S = 1 << sh;
     const double2 *w = a.tw + a.off[d];
@@ -186,15 +258,7 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
         const int s = o & (S - 1), kk = o >> sh;
         const int k = kk >= m ? kk - m : kk;
         const int base = s + 2 * S * k;
-        const double2 x0 = buf[base * CB + cb];
-        const double2 x1 = buf[(base + S) * CB + cb];
-        const double2 t1 = w[kk];
-        double re = 0.0, im = 0.0;
-        re += x0.x * w0.x - x0.y * w0.y;
-        im += x0.x * w0.y + x0.y * w0.x;
-        re += x1.x * t1.x - x1.y * t1.y;
-        im += x1.x * t1.y + x1.y * t1.x;
-        out[u] = make_double2(re, im);
+        out[u] = bfly2(buf[base * CB + cb], buf[(base + S) * CB + cb], w0, w[kk]);
       }
     }
     __syncthreads();
@@ -204,46 +268,50 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
       if (e < E) buf[e] = out[u];
     }
     __syncthreads();
+    d--;
+  }
+  // pairs of levels: lower d (L/2, 2S), upper d - 1 (L, S); one group of
+  // four per thread (E / 4 <= FFT2_TPB)
+  for (; d >= 1; d -= 2) {
+    const int L = a.L[d - 1], sh = a.sh[d - 1], q = L >> 2;  // q = L/4
+    const int S = 1 << sh;
+    const double2 *wl = a.tw + a.off[d], *wu = a.tw + a.off[d - 1];
+    const double2 wl0 = wl[0], wu0 = wu[0];
+    const bool live = tid < (E >> 2);
+    const int cb = tid & (CB - 1), rest = tid >> LCB;  // group: line cb, sub-problem s, k2
+    const int s = rest & (S - 1), k2 = rest >> sh;
+    double2 z0 = make_double2(0.0, 0.0), z1 = z0, z2 = z0, z3 = z0;
+    if (live) {
+      const int b = s + 4 * S * k2;
+      const double2 x0 = buf[b * CB + cb], x1 = buf[(b + S) * CB + cb];
+      const double2 x2 = buf[(b + 2 * S) * CB + cb], x3 = buf[(b + 3 * S) * CB + cb];
+      // lower level: sub-problem s (x0, x2) and s + S (x1, x3) at k2, k2 + L/4
+      const double2 ta = wl[k2], tb = wl[k2 + q];
+      const double2 y0 = bfly2(x0, x2, wl0, ta), y1 = bfly2(x0, x2, wl0, tb);
+      const double2 y2 = bfly2(x1, x3, wl0, ta), y3 = bfly2(x1, x3, wl0, tb);
+      // upper level: outputs k2 + i L/4 of sub-problem s
+      z0 = bfly2(y0, y2, wu0, wu[k2]);
+      z1 = bfly2(y1, y3, wu0, wu[k2 + q]);
+      z2 = bfly2(y0, y2, wu0, wu[k2 + 2 * q]);
+      z3 = bfly2(y1, y3, wu0, wu[k2 + 3 * q]);
+    }
+    __syncthreads();
+    if (live) {
+      const int o = s + S * k2;
+      buf[o * CB + cb] = z0;
+      buf[(o + S * q) * CB + cb] = z1;
+      buf[(o + 2 * S * q) * CB + cb] = z2;
+      buf[(o + 3 * S * q) * CB + cb] = z3;
+    }
+    __syncthreads();
   }
   if (a.elem_stride == 1) {
 #pragma unroll
-    for (int cb = 0; cb < CB; cb++) {
-      const int line = line0 + cb;
-      if (line >= a.nlines) continue;
-      double2 *dst = data + (long)line * a.line_step;
-      for (int o = tid; o < N; o += FFT2_TPB) dst[o] = buf[o * CB + cb];
-    }
+    for (int cb = 0; cb < CB; cb++)
+      for (int o = tid; o < N; o += FFT2_TPB) fft_store<MODE>(a, g, line0 + cb, o, buf[o * CB + cb]);
   } else {
-    for (int e = tid; e < E; e += FFT2_TPB) {
-      const int cb = e & (CB - 1), o = e >> LCB, line = line0 + cb;
-      if (line < a.nlines) data[line * a.line_step + o * a.elem_stride] = buf[e];
-    }
+    for (int e = tid; e < E; e += FFT2_TPB) fft_store<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB, buf[e]);
   }
-}
-
-// Sigma of cell l after the substep's first damping pass (the density part
-// of EvanescentBoundaryKernel, Kernels.cu:978-1004; the fused path applies
-// that pass in its source kernel, after the self-gravity field it needs)
-__device__ __forceinline__ double sigma_damped(const fg_dev_tables &t, int i, double dt, int damp, double s) {
-  if (damp && t.ev_zone[i]) s = damp_toward(s, t.ev_coef[i] * dt / t.ev_period[i], t.SigmaMed[i]);
-  return s;
-}
-
-// ComputeFFTKernel, Kernels.cu:840-868 (density half of the padded grids);
-// damp: Sigma as after the first damping pass of a substep of length dt
-__global__ void k_sg_fill(double2 *Sr, double2 *St, const double *dens, fg_dev_tables t, double dt, int damp,
-                          int nr, int ns) {
-  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= 2L * nr * ns) return;
-  const int i = (int)(l / ns);
-  double sr = 0.0, st = 0.0;
-  if (i < nr) {
-    const double var = sigma_damped(t, i, dt, damp, dens[l]) * t.sg_sqrt_ratio[i];
-    sr = var;
-    st = var * t.Rmed[i] / t.Rmed[0];
-  }
-  Sr[l] = make_double2(sr, 0.0);
-  St[l] = make_double2(st, 0.0);
 }
 
 __global__ void k_sg_kernel_fill(double2 *Kr, double2 *Kt, fg_dev_tables t, long n) {
@@ -253,71 +321,42 @@ __global__ void k_sg_kernel_fill(double2 *Kr, double2 *Kt, fg_dev_tables t, long
   Kt[l] = make_double2(t.sg_Kt[l], 0.0);
 }
 
-// ComputeConvolutionKernel, Kernels.cu:871-890 (G = 1)
-__global__ void k_sg_convolve(double2 *Gr, double2 *Gt, const double2 *Kr, const double2 *Kt, const double2 *Sr,
-                              const double2 *St, long n) {
-  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= n) return;
-  const double2 kt = Kt[l], st = St[l], kr = Kr[l], sr = Sr[l];
-  Gt[l] = make_double2(-1.0 * (kt.x * st.x - kt.y * st.y), -1.0 * (kt.x * st.y + kt.y * st.x));
-  Gr[l] = make_double2(-1.0 * (kr.x * sr.x - kr.y * sr.y), -1.0 * (kr.x * sr.y + kr.y * sr.x));
-}
-
-// ComputeSgAccKernel, Kernels.cu:893-912
-__global__ void k_sg_acc(double *accr, double *acct, const double *dens, const double2 *Gr, const double2 *Gt,
-                         fg_dev_tables t, double rstep, double tstep, double eps, int nr, int ns) {
-  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= (long)nr * ns) return;
-  const int i = (int)(l / ns);
-  acct[l] = Gt[l].x * t.sg_norm_t[i];
-  double a = Gr[l].x * t.sg_norm_r[i];
-  a += 1.0 * dens[l] * rstep * tstep / eps;
-  accr[l] = a;
-}
-
-// The fused path's ComputeSgAccKernel: the accelerations from the density
-// after the first damping pass, and the kick that Update_sgvelocityKernel
-// (Kernels.cu:915-932) adds to VradInt / VthetaInt, for the source kernel
-// to add (0 on ring 0, which the kick leaves alone).  The neighbours'
-// accelerations are recomputed from the same expressions.
-__global__ void k_sg_acc_fused(double *accr, double *acct, double *kick_r, double *kick_t, const double *dens,
-                               const double2 *Gr, const double2 *Gt, fg_dev_tables t, double rstep, double tstep,
-                               double eps, double dt, int damp, int nr, int ns) {
-  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= (long)nr * ns) return;
-  const int i = (int)(l / ns), j = (int)(l - (long)i * ns);
+// ComputeSgAccKernel (Kernels.cu:893-912) and the ring means of g_r
+// (Make1Dprofile, Kernels.cu:610-623: the sectors summed in j order) for
+// one ring per workgroup.  damp: the density after the first damping pass
+// of a substep of length dt.  kick_r / kick_t (the fused path; NULL on the
+// per-routine path, which adds them with k_sg_kick): the increments
+// Update_sgvelocityKernel (Kernels.cu:915-932) adds to VradInt /
+// VthetaInt, 0 for VradInt on ring 0, which it leaves alone; the
+// neighbours' accelerations are recomputed with the same expressions.
+__global__ __launch_bounds__(256) void k_sg_acc_ring(double *accr, double *acct, double *kick_r, double *kick_t,
+                                                     double *mean, const double *dens, const double2 *Gr,
+                                                     const double2 *Gt, fg_dev_tables t, double rstep,
+                                                     double tstep, double eps, double dt, int damp, int ns) {
+  extern __shared__ double ring_acc[];
+  const int i = blockIdx.x;
+  const long base = (long)i * ns;
   auto acc_r = [&](long m, int ii) {
     double a = Gr[m].x * t.sg_norm_r[ii];
     a += 1.0 * sigma_damped(t, ii, dt, damp, dens[m]) * rstep * tstep / eps;
     return a;
   };
-  const long lm = (long)i * ns + (j == 0 ? ns - 1 : j - 1);
-  const double ar = acc_r(l, i), at = Gt[l].x * t.sg_norm_t[i], atm = Gt[lm].x * t.sg_norm_t[i];
-  accr[l] = ar;
-  acct[l] = at;
-  kick_r[l] = i > 0 ? dt * ((t.Rinf[i] - t.Rmed[i - 1]) * ar + (t.Rmed[i] - t.Rinf[i]) * acc_r(l - ns, i - 1)) *
-                          t.invdiffRmed[i]
-                    : 0.0;
-  kick_t[l] = 0.5 * dt * (at + atm);
-}
-
-// Make1Dprofile of SG_Accr: sequential sum over the sectors of each ring
-// by lane 0 of one wave, streamed through LDS
-__global__ __launch_bounds__(64) void k_sg_ring_mean(const double *accr, double *mean, int ns) {
-  __shared__ double chunk[64];
-  const int lane = threadIdx.x;
-  const double *v = accr + (long)blockIdx.x * ns;
-  double sum = 0.0;
-  for (int c0 = 0; c0 < ns; c0 += 64) {
-    chunk[lane] = (c0 + lane < ns) ? v[c0 + lane] : 0.0;
-    __syncthreads();
-    if (lane == 0) {
-      const int n = min(64, ns - c0);
-      for (int k = 0; k < n; k++) sum += chunk[k];
+  for (int j = threadIdx.x; j < ns; j += 256) {
+    const long l = base + j;
+    const double ar = acc_r(l, i), at = Gt[l].x * t.sg_norm_t[i];
+    accr[l] = ar;
+    acct[l] = at;
+    ring_acc[j] = ar;
+    if (kick_r) {
+      const double atm = Gt[base + (j == 0 ? ns - 1 : j - 1)].x * t.sg_norm_t[i];
+      kick_r[l] = i > 0 ? dt * ((t.Rinf[i] - t.Rmed[i - 1]) * ar + (t.Rmed[i] - t.Rinf[i]) * acc_r(l - ns, i - 1)) *
+                              t.invdiffRmed[i]
+                        : 0.0;
+      kick_t[l] = 0.5 * dt * (at + atm);
     }
-    __syncthreads();
   }
-  if (lane == 0) mean[blockIdx.x] = sum / (double)ns;
+  __syncthreads();
+  if (threadIdx.x == 0) mean[i] = seq_sum_lds(ring_acc, ns) / (double)ns;
 }
 
 // Update_sgvelocityKernel, Kernels.cu:915-932, on VradInt / VthetaInt;
@@ -391,55 +430,96 @@ void fg_sg_release(fg_ctx *c) {
   }
 }
 
-// one 2D transform (rows, then columns) of one or two 2NRAD x NSEC grids
-static fg_status fft2d(fg_ctx *c, double *g0, double *g1, int inverse) {
+// one pass (rows: pass 0, columns: pass 1) of a 2D transform of one or two
+// 2NRAD x NSEC grids; a mode other than FFT_PLAIN also forms the input or
+// the output (fft_load / fft_store): FFT_FILL transforms the NRAD rows of
+// the density, FFT_CONV and FFT_HEAD take zrows = NRAD
+template <int MODE>
+static void fft_launch(fg_ctx *c, const fft_args &a, const fg_fft_plan *pl, int ngrids) {
+  const int blocks = (a.nlines + a.lines_per_block - 1) / a.lines_per_block;
+  const size_t shm = (size_t)a.n * a.lines_per_block * sizeof(double2);
+  const dim3 grid(blocks, ngrids);
+  if (!pl->pow2s)
+    hipLaunchKernelGGL(k_fft_lines<MODE>, grid, dim3(FFT_TPB), shm, c->stream, a);
+  else if (a.lines_per_block == 8)
+    hipLaunchKernelGGL((k_fft_pow2<8, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+  else if (a.lines_per_block == 4)
+    hipLaunchKernelGGL((k_fft_pow2<4, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+  else if (a.lines_per_block == 2)
+    hipLaunchKernelGGL((k_fft_pow2<2, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_fft_pow2<1, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+}
+
+static fg_status fft_pass(fg_ctx *c, int pass, int inverse, int mode, double *g0, double *g1, double dt = 0.0,
+                          int damp = 0, double *out0 = nullptr, double *out1 = nullptr) {
   const int n1 = 2 * c->nr, n2 = c->ns;
   const int ngrids = g1 ? 2 : 1;
-  for (int pass = 0; pass < 2; pass++) {
-    const fg_fft_plan *pl = pass == 0 ? c->fft_row : c->fft_col;
-    fft_args a;
-    a.nst = pl->nst;
-    for (int d = 0; d < pl->nst; d++) {
-      a.p[d] = pl->p[d];
-      a.L[d] = pl->L[d];
-      a.S[d] = pl->S[d];
-      a.off[d] = pl->off[d];
-      a.sh[d] = pl->sh[d];
-    }
-    a.tw = pl->tw[inverse ? 1 : 0];
-    a.data[0] = (double2 *)g0;
-    a.data[1] = (double2 *)(g1 ? g1 : g0);
-    a.n = pl->n;
-    // lines staged per workgroup: up to FFT_MAX_ELEMS elements, a power of
-    // two up to 8 for k_fft_pow2 (rows 2 at a time, columns 4 at NSEC 1536)
-    int cb = FFT_MAX_ELEMS / pl->n;
-    cb = cb >= 8 ? 8 : cb >= 4 ? 4 : cb >= 2 ? 2 : 1;
-    if (pass == 0) {  // rows: contiguous
-      a.nlines = n1;
-      a.elem_stride = 1;
-      a.line_step = n2;
-      a.lines_per_block = pl->pow2s ? (cb > 2 ? 2 : cb) : 1;
-    } else {  // columns: stride NSEC, adjacent columns staged together
-      a.nlines = n2;
-      a.elem_stride = n2;
-      a.line_step = 1;
-      a.lines_per_block = cb;
-    }
-    const int blocks = (a.nlines + a.lines_per_block - 1) / a.lines_per_block;
-    const size_t shm = (size_t)a.n * a.lines_per_block * sizeof(double2);
-    if (!pl->pow2s)
-      hipLaunchKernelGGL(k_fft_lines, dim3(blocks, ngrids), dim3(FFT_TPB), shm, c->stream, a);
-    else if (a.lines_per_block == 8)
-      hipLaunchKernelGGL(k_fft_pow2<8>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
-    else if (a.lines_per_block == 4)
-      hipLaunchKernelGGL(k_fft_pow2<4>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
-    else if (a.lines_per_block == 2)
-      hipLaunchKernelGGL(k_fft_pow2<2>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
-    else
-      hipLaunchKernelGGL(k_fft_pow2<1>, dim3(blocks, ngrids), dim3(FFT2_TPB), shm, c->stream, a);
-    FG_LAUNCH_CHECK();
+  const fg_fft_plan *pl = pass == 0 ? c->fft_row : c->fft_col;
+  fft_args a;
+  a.nst = pl->nst;
+  for (int d = 0; d < pl->nst; d++) {
+    a.p[d] = pl->p[d];
+    a.L[d] = pl->L[d];
+    a.S[d] = pl->S[d];
+    a.off[d] = pl->off[d];
+    a.sh[d] = pl->sh[d];
   }
+  a.tw = pl->tw[inverse ? 1 : 0];
+  a.data[0] = (double2 *)g0;
+  a.data[1] = (double2 *)(g1 ? g1 : g0);
+  a.n = pl->n;
+  a.dens = c->f[FG_DENS];
+  a.t = c->t;
+  a.dt = dt;
+  a.damp = damp;
+  a.zrows = c->nr;
+  a.kern[0] = (const double2 *)c->sg_Ktc;
+  a.kern[1] = (const double2 *)c->sg_Krc;
+  a.out[0] = (double2 *)out0;
+  a.out[1] = (double2 *)(out1 ? out1 : out0);
+  // lines staged per workgroup: up to FFT_MAX_ELEMS elements, a power of
+  // two up to 8 for k_fft_pow2 (rows 2 at a time, columns 4 at NSEC 1536)
+  int cb = FFT_MAX_ELEMS / pl->n;
+  cb = cb >= 8 ? 8 : cb >= 4 ? 4 : cb >= 2 ? 2 : 1;
+  if (pass == 0) {  // rows: contiguous
+    a.nlines = mode == FFT_FILL ? c->nr : n1;
+    a.elem_stride = 1;
+    a.line_step = n2;
+    a.lines_per_block = pl->pow2s ? (cb > 2 ? 2 : cb) : 1;
+  } else {  // columns: stride NSEC, adjacent columns staged together
+    a.nlines = n2;
+    a.elem_stride = n2;
+    a.line_step = 1;
+    a.lines_per_block = cb;
+  }
+  switch (mode) {
+    case FFT_FILL: fft_launch<FFT_FILL>(c, a, pl, ngrids); break;
+    case FFT_CONV: fft_launch<FFT_CONV>(c, a, pl, ngrids); break;
+    case FFT_HEAD: fft_launch<FFT_HEAD>(c, a, pl, ngrids); break;
+    default: fft_launch<FFT_PLAIN>(c, a, pl, ngrids);
+  }
+  FG_LAUNCH_CHECK();
   return FG_OK;
+}
+
+// one 2D transform (rows, then columns) of one or two 2NRAD x NSEC grids
+static fg_status fft2d(fg_ctx *c, double *g0, double *g1, int inverse) {
+  if (fg_status s = fft_pass(c, 0, inverse, FFT_PLAIN, g0, g1)) return s;
+  return fft_pass(c, 1, inverse, FFT_PLAIN, g0, g1);
+}
+
+// The potential's spectra G = -K * FFT(S) and their inverse transforms of
+// the density (after the first damping pass of a substep of length dt when
+// damp): the steps of compute_selfgravity (SgMain.cu:23-45) with the fill
+// in the forward row pass, the convolution in the forward column pass, and
+// the padding rows neither transformed forward nor stored backward.  Grid
+// pairs are (azimuthal, radial).
+static fg_status sg_field(fg_ctx *c, double dt, int damp) {
+  if (fg_status s = fft_pass(c, 0, 0, FFT_FILL, c->sg_St, c->sg_Sr, dt, damp)) return s;
+  if (fg_status s = fft_pass(c, 1, 0, FFT_CONV, c->sg_St, c->sg_Sr, 0.0, 0, c->sg_Gt, c->sg_Gr)) return s;
+  if (fg_status s = fft_pass(c, 0, 1, FFT_PLAIN, c->sg_Gt, c->sg_Gr)) return s;
+  return fft_pass(c, 1, 1, FFT_HEAD, c->sg_Gt, c->sg_Gr);
 }
 
 static fg_status sg_prepare(fg_ctx *c) {
@@ -470,24 +550,14 @@ extern "C" fg_status fg_selfgravity(fg_ctx *c, double dt, int update) {
   }
   if (fg_status s = sg_prepare(c)) return s;
   const int nr = c->nr, ns = c->ns;
-  const long n = 2L * nr * ns, z = (long)nr * ns;
-  hipLaunchKernelGGL(k_sg_fill, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Sr,
-                     (double2 *)c->sg_St, c->f[FG_DENS], c->t, 0.0, 0, nr, ns);
-  FG_LAUNCH_CHECK();
-  if (fg_status s = fft2d(c, c->sg_St, c->sg_Sr, 0)) return s;
-  hipLaunchKernelGGL(k_sg_convolve, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Gr,
-                     (double2 *)c->sg_Gt, (const double2 *)c->sg_Krc, (const double2 *)c->sg_Ktc,
-                     (const double2 *)c->sg_Sr, (const double2 *)c->sg_St, n);
-  FG_LAUNCH_CHECK();
-  if (fg_status s = fft2d(c, c->sg_Gt, c->sg_Gr, 1)) return s;
-  hipLaunchKernelGGL(k_sg_acc, dim3(fg_blocks(z, 256)), dim3(256), 0, c->stream, c->f[FG_SG_ACCR],
-                     c->f[FG_SG_ACCT], c->f[FG_DENS], (const double2 *)c->sg_Gr, (const double2 *)c->sg_Gt, c->t,
-                     c->p.sg_rstep, c->p.sg_tstep, c->p.sg_eps, nr, ns);
-  FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sg_ring_mean, dim3(nr), dim3(64), 0, c->stream, c->f[FG_SG_ACCR], c->sgacc_mean, ns);
+  if (fg_status s = sg_field(c, 0.0, 0)) return s;
+  hipLaunchKernelGGL(k_sg_acc_ring, dim3(nr), dim3(256), ns * sizeof(double), c->stream, c->f[FG_SG_ACCR],
+                     c->f[FG_SG_ACCT], (double *)nullptr, (double *)nullptr, c->sgacc_mean, c->f[FG_DENS],
+                     (const double2 *)c->sg_Gr, (const double2 *)c->sg_Gt, c->t, c->p.sg_rstep, c->p.sg_tstep,
+                     c->p.sg_eps, 0.0, 0, ns);
   FG_LAUNCH_CHECK();
   if (update) {
-    hipLaunchKernelGGL(k_sg_kick, dim3(fg_blocks(z, 256)), dim3(256), 0, c->stream, c->f[FG_VRADINT],
+    hipLaunchKernelGGL(k_sg_kick, dim3(fg_blocks((long)nr * ns, 256)), dim3(256), 0, c->stream, c->f[FG_VRADINT],
                        c->f[FG_VTHETAINT], c->f[FG_SG_ACCR], c->f[FG_SG_ACCT], c->t, dt, nr, ns);
     FG_LAUNCH_CHECK();
   }
@@ -503,23 +573,12 @@ fg_status fg_selfgravity_kick(fg_ctx *c, double dt) { return fg_selfgravity(c, d
 fg_status fg_sg_fused(fg_ctx *c, double dt, double *kick_r, double *kick_t) {
   if (fg_status s = sg_prepare(c)) return s;
   const int nr = c->nr, ns = c->ns;
-  const long n = 2L * nr * ns, z = (long)nr * ns;
   const int damp = c->p.evanescent;
   std::swap(c->sgacc_mean, c->sgacc_prev);
-  hipLaunchKernelGGL(k_sg_fill, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Sr,
-                     (double2 *)c->sg_St, c->f[FG_DENS], c->t, dt, damp, nr, ns);
-  FG_LAUNCH_CHECK();
-  if (fg_status s = fft2d(c, c->sg_St, c->sg_Sr, 0)) return s;
-  hipLaunchKernelGGL(k_sg_convolve, dim3(fg_blocks(n, 256)), dim3(256), 0, c->stream, (double2 *)c->sg_Gr,
-                     (double2 *)c->sg_Gt, (const double2 *)c->sg_Krc, (const double2 *)c->sg_Ktc,
-                     (const double2 *)c->sg_Sr, (const double2 *)c->sg_St, n);
-  FG_LAUNCH_CHECK();
-  if (fg_status s = fft2d(c, c->sg_Gt, c->sg_Gr, 1)) return s;
-  hipLaunchKernelGGL(k_sg_acc_fused, dim3(fg_blocks(z, 256)), dim3(256), 0, c->stream, c->f[FG_SG_ACCR],
-                     c->f[FG_SG_ACCT], kick_r, kick_t, c->f[FG_DENS], (const double2 *)c->sg_Gr,
-                     (const double2 *)c->sg_Gt, c->t, c->p.sg_rstep, c->p.sg_tstep, c->p.sg_eps, dt, damp, nr, ns);
-  FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sg_ring_mean, dim3(nr), dim3(64), 0, c->stream, c->f[FG_SG_ACCR], c->sgacc_mean, ns);
+  if (fg_status s = sg_field(c, dt, damp)) return s;
+  hipLaunchKernelGGL(k_sg_acc_ring, dim3(nr), dim3(256), ns * sizeof(double), c->stream, c->f[FG_SG_ACCR],
+                     c->f[FG_SG_ACCT], kick_r, kick_t, c->sgacc_mean, c->f[FG_DENS], (const double2 *)c->sg_Gr,
+                     (const double2 *)c->sg_Gt, c->t, c->p.sg_rstep, c->p.sg_tstep, c->p.sg_eps, dt, damp, ns);
   FG_LAUNCH_CHECK();
   return FG_OK;
 }
