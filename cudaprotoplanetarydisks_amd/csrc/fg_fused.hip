@@ -2168,6 +2168,19 @@ fg_status fg_fused_check(fg_ctx *c, double cfl, const double *force2) {
   return fg_check_substep_results(cfl, force2);
 }
 
+fg_status fg_fused_planet_force_async(fg_ctx *c, int k, double x, double y, double rs) {
+  if (!c->d_fused || k < 0 || k >= FG_MAX_PLANETS) {
+    fg_set_error("fused planet force: no result block or planet %d", k);
+    return FG_E_INVALID;
+  }
+  return fg_disk_force_async(c, c->f[FG_DENS], x, y, rs, c->d_fused + FG_FUSED_PF_SLOT + 2 * k);
+}
+
+void fg_fused_planet_force(const fg_ctx *c, int k, double out2[2]) {
+  out2[0] = c->h_fused[FG_FUSED_PF_SLOT + 2 * k];
+  out2[1] = c->h_fused[FG_FUSED_PF_SLOT + 2 * k + 1];
+}
+
 extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, double *lost) {
   if (!c) return FG_E_INVALID;
   if (!c->h_fused) {

@@ -24,6 +24,10 @@
 // error channel of fg_last_error (fg_context.hip)
 void fg_set_error(const char *fmt, ...);
 #define host_error fg_set_error
+// a planet's disk force queued behind a fused substep, and its result once
+// the fused results are in (fg_fused.hip)
+fg_status fg_fused_planet_force_async(fg_ctx *c, int k, double x, double y, double rs);
+void fg_fused_planet_force(const fg_ctx *c, int k, double out2[2]);
 
 namespace {
 
@@ -295,6 +299,7 @@ struct fg_sim {
   long nsub = 0;          // substeps taken since creation
   int planets_fresh = 0;  // planetK.dat emptied (first output) or kept (restart)
   int results_ready = 0;  // fused CFL count / star force are current
+  int pf_ready = 0;       // the planets' disk forces for the next fused substep are in the result block
   // bumped by every change of the state outside a batch launch (set_state,
   // restart, a single-disk substep); a batch re-derives the CFL count and
   // star force of a disk whose generation it has not seen
@@ -1046,11 +1051,16 @@ static fg_status fused_plan(fg_sim *s, int cfl, const double force[2], fused_lau
   s->IndirectY = d.Indirect_Term ? -force[1] : 0.0;
   L.nb = potential_planets(s, L.pl);
   Planets &p = s->ps;
+  const bool pf = s->pf_ready;
+  s->pf_ready = 0;
   for (int k = 0; k < p.nb; k++) {  // AdvanceSystemFromDisk, Pframeforce.cu:97-121
     if (!p.feeldisk[k]) continue;
     double m = p.mass[k], x = p.x[k], y = p.y[k], r = sqrt(x * x + y * y);
     double gamma[2];
-    CHECK(fg_disk_force(c, x, y, planet_smoothing(s, r, m, true), gamma));
+    if (pf)  // launched behind the last substep, for these positions and this density
+      fg_fused_planet_force(c, k, gamma);
+    else
+      CHECK(fg_disk_force(c, x, y, planet_smoothing(s, r, m, true), gamma));
     p.vx[k] += dt * gamma[0];
     p.vy[k] += dt * gamma[1];
     p.vx[k] += dt * s->IndirectX;
@@ -1087,6 +1097,7 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
   if (!s->results_ready) {
     CHECK(fg_fused_prepare(c, d.DT - s->dtemp));
     s->results_ready = 1;
+    s->pf_ready = 0;
   }
   int cfl = 1;
   double force[2];
@@ -1095,6 +1106,21 @@ static fg_status fused_substep(fg_sim *s, int *closed) {
   CHECK(fused_plan(s, cfl, force, L));
   CHECK(fg_step_fused(c, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY, L.DeltaT_next,
                       (d.Write_Temperature ? FG_STEP_TEMPERATURE : 0) | (L.done ? FG_STEP_CLOSES_DT : 0)));
+  // the next substep's AdvanceSystemFromDisk reads the density this one
+  // leaves at the planets' positions now (fused_plan moved them): its
+  // reductions queue behind this substep and come back with its results,
+  // so the next substep waits for the GPU once
+  {
+    const Planets &p = s->ps;
+    int any = 0;
+    for (int k = 0; k < p.nb; k++) {
+      if (!p.feeldisk[k]) continue;
+      const double m = p.mass[k], x = p.x[k], y = p.y[k], r = sqrt(x * x + y * y);
+      CHECK(fg_fused_planet_force_async(c, k, x, y, planet_smoothing(s, r, m, true)));
+      any = 1;
+    }
+    s->pf_ready = any;
+  }
   CHECK(substep_diagnostics(s));
   fused_close(s, L, closed);
   return FG_OK;
@@ -1639,6 +1665,7 @@ extern "C" fg_status fg_batch_substep(fg_batch *b, double t_end, int *n_active) 
     na++;
     fused_open(s);
     fused_launch &L = b->L[k];
+    s->pf_ready = 0;  // batch disks take their planets' forces synchronously
     CHECK(fused_plan(s, b->cfl[k], &b->force[2 * k], L));
     CHECK(fg_batch_dev_set(b->dev, k, 1, L.dt, s->OmegaFrame, L.domega, L.pl, L.nb, s->IndirectX, s->IndirectY,
                            L.DeltaT_next));

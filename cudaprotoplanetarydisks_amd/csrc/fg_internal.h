@@ -31,10 +31,18 @@ fg_status fg_viscosity_into(fg_ctx *c, double *nu_out);
 // memory the final kernels write straight into): [0..1] star force,
 // FG_FUSED_CFL_SLOT the CFL count (as a double), FG_FUSED_CRASH_SLOT an int
 // that k_fin and the CFL ring pass set to 1 when they meet a negative or
-// non-finite density.
-#define FG_FUSED_SLOTS 8
+// non-finite density, and from FG_FUSED_PF_SLOT the disk force on each
+// planet that feels the disk (two doubles per planet), launched behind a
+// substep for the next one.
+#define FG_FUSED_PF_SLOT 8
+#define FG_FUSED_SLOTS (FG_FUSED_PF_SLOT + 2 * FG_MAX_PLANETS)
 #define FG_FUSED_CFL_SLOT 6
 #define FG_FUSED_CRASH_SLOT 7
+// disk force on planet k at (x, y) with smoothing rs from the current
+// density into the result block (fg_fused.hip), read once the fused
+// results are in (fg_fused_results)
+fg_status fg_fused_planet_force_async(fg_ctx *c, int k, double x, double y, double rs);
+void fg_fused_planet_force(const fg_ctx *c, int k, double out2[2]);
 
 // Largest CFL sub-cycle count accepted per DT.  The count is
 // ceil(DeltaT / dt) (SourceEuler.cu:228); a count outside 1..FG_CFL_MAX

@@ -108,11 +108,24 @@ __device__ __forceinline__ void fft_store(const fft_args &a, int g, int line, in
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs.  A column pass
+// reads each grid row as CB-column pieces of one 128-byte line shared with
+// the neighbouring column groups, so consecutive column groups go to the
+// same XCD (and L2): workgroup b takes group (b % 8) * per + b / 8 of a
+// grid of 8 * per workgroups.  Rows are contiguous and keep b.
+__device__ __forceinline__ int fft_block_line(const fft_args &a) {
+  const int b = blockIdx.x;
+  if (a.elem_stride == 1) return b;
+  const int per = gridDim.x >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
   extern __shared__ double2 buf[];
   const int CB = a.lines_per_block, N = a.n, tid = threadIdx.x;
-  const int line0 = blockIdx.x * CB, g = blockIdx.y;
+  const int line0 = fft_block_line(a) * CB, g = blockIdx.y;
+  if (line0 >= a.nlines) return;  // (whole workgroup)
   const int E = N * CB;
   // stage: element o of line cb at buf[o*CB + cb]
   for (int e = tid; e < E; e += FFT_TPB) {
@@ -198,7 +211,8 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
   extern __shared__ double2 buf[];
   constexpr int LCB = CB == 1 ? 0 : CB == 2 ? 1 : CB == 4 ? 2 : 3;
   const int N = a.n, tid = threadIdx.x;
-  const int line0 = blockIdx.x * CB, g = blockIdx.y;
+  const int line0 = fft_block_line(a) * CB, g = blockIdx.y;
+  if (line0 >= a.nlines) return;  // (whole workgroup)
   const int E = N * CB;
   if (a.elem_stride == 1) {
 #pragma unroll
@@ -342,21 +356,29 @@ __global__ __launch_bounds__(256) void k_sg_acc_ring(double *accr, double *acct,
     return a;
   };
   for (int j = threadIdx.x; j < ns; j += 256) {
-    const long l = base + j;
-    const double ar = acc_r(l, i), at = Gt[l].x * t.sg_norm_t[i];
-    accr[l] = ar;
-    acct[l] = at;
+    const double ar = acc_r(base + j, i);
+    accr[base + j] = ar;
     ring_acc[j] = ar;
-    if (kick_r) {
-      const double atm = Gt[base + (j == 0 ? ns - 1 : j - 1)].x * t.sg_norm_t[i];
-      kick_r[l] = i > 0 ? dt * ((t.Rinf[i] - t.Rmed[i - 1]) * ar + (t.Rmed[i] - t.Rinf[i]) * acc_r(l - ns, i - 1)) *
-                              t.invdiffRmed[i]
-                        : 0.0;
-      kick_t[l] = 0.5 * dt * (at + atm);
-    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) mean[i] = seq_sum_lds(ring_acc, ns) / (double)ns;
+  // lane 0 sums the ring while waves 1-3 form g_theta and the kicks
+  if (threadIdx.x == 0) {
+    mean[i] = seq_sum_lds(ring_acc, ns) / (double)ns;
+  } else if (threadIdx.x >= 64) {
+    for (int j = threadIdx.x - 64; j < ns; j += 192) {
+      const long l = base + j;
+      const double at = Gt[l].x * t.sg_norm_t[i];
+      acct[l] = at;
+      if (kick_r) {
+        const double atm = Gt[base + (j == 0 ? ns - 1 : j - 1)].x * t.sg_norm_t[i];
+        kick_r[l] = i > 0 ? dt * ((t.Rinf[i] - t.Rmed[i - 1]) * ring_acc[j] +
+                                  (t.Rmed[i] - t.Rinf[i]) * acc_r(l - ns, i - 1)) *
+                                t.invdiffRmed[i]
+                          : 0.0;
+        kick_t[l] = 0.5 * dt * (at + atm);
+      }
+    }
+  }
 }
 
 // Update_sgvelocityKernel, Kernels.cu:915-932, on VradInt / VthetaInt;
@@ -436,7 +458,8 @@ void fg_sg_release(fg_ctx *c) {
 // the density, FFT_CONV and FFT_HEAD take zrows = NRAD
 template <int MODE>
 static void fft_launch(fg_ctx *c, const fft_args &a, const fg_fft_plan *pl, int ngrids) {
-  const int blocks = (a.nlines + a.lines_per_block - 1) / a.lines_per_block;
+  int blocks = (a.nlines + a.lines_per_block - 1) / a.lines_per_block;
+  if (a.elem_stride != 1) blocks = 8 * ((blocks + 7) / 8);  // column groups in XCD order (fft_block_line)
   const size_t shm = (size_t)a.n * a.lines_per_block * sizeof(double2);
   const dim3 grid(blocks, ngrids);
   if (!pl->pow2s)
