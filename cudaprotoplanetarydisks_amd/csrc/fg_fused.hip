@@ -21,8 +21,13 @@
 //           fg_reduce.hip) and the CFL count in one final kernel
 //           (k_force_cfl_final; FP32 mode: k_force_rings_cfl)
 //
-// Supported decks: locally isothermal, evanescent boundaries, no
-// self-gravity, per-DT CFL (everything else uses the per-routine path).
+// Adiabatic decks add Substep3 (k_src3) between k_src and k_trad and carry
+// the energy through the sweeps and k_fin; self-gravitating decks run the
+// SG field (fg_sg_fused) first and add its kick in k_src.
+//
+// Supported decks: evanescent boundaries, per-substep CFL; isothermal or
+// adiabatic, with or without self-gravity (FP32 bulk mode: isothermal
+// without self-gravity).  Everything else uses the per-routine path.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
