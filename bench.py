@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 vector
 SEED = 20261202
 METRIC = "zone-updates/s + % HBM peak; ensemble time-to-steady-gap, emulator vs plain"
 KERNELS = ["k_src", "k_trad", "k_taz", "k_fin", "force_reduce", "cfl_count"]
@@ -252,9 +253,21 @@ def roofline_of(prof, nr, ns, precision):
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get(f"{precision}:{dom}")
-    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+    out = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+    # the compute side of the same launch: executed FP64 operations per launch
+    # (rocprofv3 SQ_INSTS_VALU_*_F64 counters at 1024 x 3072, committed in
+    # profiles/fp64_ops.json) over the live event-timed duration
+    ofile = os.path.join(ROOT, "profiles", "fp64_ops.json")
+    if precision == "fp64" and (nr, ns) == (1024, 3072) and os.path.exists(ofile):
+        with open(ofile) as f:
+            ops = json.load(f).get(f"fp64:{dom}")
+        if ops:
+            tf = ops / (prof[dom] * 1e-3) / 1e12
+            out["fp64"] = {"ops_per_launch": ops, "achieved": round(tf, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                           "frac": round(tf / FP64_PEAK_TFS, 4)}
+    return out
 
 
 def main():
