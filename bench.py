@@ -32,17 +32,28 @@ METRIC = "zone-updates/s + % HBM peak; ensemble time-to-steady-gap, emulator vs 
 KERNELS = ["k_src", "k_trad", "k_taz", "k_fin", "force_reduce", "cfl_count"]
 
 
-def algorithmic_bytes(nr, ns, word):
+def algorithmic_bytes(nr, ns, precision):
     """Compulsory HBM bytes of one launch of each fused kernel: every
     input field read once and every output field written once, over the
-    (NRAD+1) x NSEC field."""
+    (NRAD+1) x NSEC field.  The state between substeps is FP64 in both
+    modes; the FP32 mode stores the fields passed between the kernels
+    (after k_src, k_trad and k_taz) in FP32."""
     S = (nr + 1) * ns
+    if precision == "fp32":
+        return {
+            "k_src": S * (3 * 8 + 3 * 4),   # FP64 state in; Sigma, vrad, vtheta out (FP32)
+            "k_trad": S * (3 * 4 + 5 * 4),  # Sigma, vrad, vtheta in; Sigma + 4 momenta out
+            "k_taz": S * (6 * 4 + 4 * 4),   # 5 quantities + vtheta in; 4 out
+            "k_fin": S * (4 * 4 + 3 * 8),   # 4 in; final FP64 state out
+            "force_reduce": nr * ns * 8,
+            "cfl_count": nr * 8 * 2,
+        }
     return {
-        "k_src": S * word * (3 + 3),   # Sigma, vrad, vtheta in and out
-        "k_trad": S * word * (3 + 5),  # state in; Sigma + 4 momenta out
-        "k_taz": S * word * (6 + 4),   # 5 quantities + vtheta in; 4 out
-        "k_fin": S * word * (4 + 3),   # 4 in; final state out
-        "force_reduce": nr * ns * word,
+        "k_src": S * 8 * (3 + 3),   # Sigma, vrad, vtheta in and out
+        "k_trad": S * 8 * (3 + 5),  # state in; Sigma + 4 momenta out
+        "k_taz": S * 8 * (6 + 4),   # 5 quantities + vtheta in; 4 out
+        "k_fin": S * 8 * (4 + 3),   # 4 in; final state out
+        "force_reduce": nr * ns * 8,
         "cfl_count": nr * 8 * 2,
     }
 
@@ -241,12 +252,11 @@ def emulator_leg(n_eval, max_orbits):
 
 def roofline_of(prof, nr, ns, precision):
     """Roofline entry of the dominant (longest) kernel: algorithmic bytes
-    per launch over its event-timed duration.  Both precisions keep the
-    state and the quantities passed between kernels in FP64."""
+    per launch (algorithmic_bytes) over its event-timed duration."""
     if not prof:
         return None
     dom = max(prof, key=prof.get)
-    ab = algorithmic_bytes(nr, ns, 8)[dom]
+    ab = algorithmic_bytes(nr, ns, precision)[dom]
     achieved = ab / (prof[dom] * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")

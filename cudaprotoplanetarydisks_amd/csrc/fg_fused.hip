@@ -58,6 +58,15 @@ struct fused_bufs {
   // adiabatic decks: EnergyInt after k_src, energy after k_trad and k_taz;
   // self-gravity: the kick of VradInt / VthetaInt (NULL when not allocated)
   double *Beint, *Re, *Xe, *kick_r, *kick_t;
+  // FP32 bulk mode: B, R and X stored in FP32 in the same arrays
+  float *fBd() const { return (float *)Bd; }
+  float *fBvr() const { return (float *)Bvr; }
+  float *fBvt() const { return (float *)Bvt; }
+  float *fR(int q) const { return (float *)(q == 0 ? Rrmp : q == 1 ? Rrmm : q == 2 ? Rtmp : q == 3 ? Rtmm : Rd); }
+  float *fXd() const { return (float *)Xd; }
+  float *fXvt() const { return (float *)Xvt; }
+  float *fXrmp() const { return (float *)Xrmp; }
+  float *fXrmm() const { return (float *)Xrmm; }
 };
 
 struct src_args {
@@ -516,6 +525,21 @@ __global__ __launch_bounds__(256) void k_src3(src3_args a) {
 // the face value at face m-1 are formed once and kept in registers; the
 // face flux F(k) = dt*dtheta*Rinf[k]*Q*(k)*Sigma*(k)*vrad(k) serves both
 // neighbouring rings (Rsup[k-1] == Rinf[k]).
+// FP32 bulk mode storage.  The fields passed between the fused kernels
+// (after k_src, k_trad and k_taz) are stored in FP32 as offsets from FP64
+// backgrounds that writer and reader form alike: Sigma from sbar (ring
+// means of the state the substep starts from), vtheta from vref (the ring
+// means of vtheta at that state, Vmoy), the angular momenta from the
+// cell's own stored Sigma times (vref + R Omega) R (fp32_bg_angmom; the
+// writer takes Sigma as rounded for the reader); vrad and the radial
+// momenta are stored as they are.  The offsets are small against the
+// fields (a few 1e-3 of vtheta, the non-axisymmetric part of Sigma), so
+// their FP32 rounding stays far below the FP32 arithmetic's own error; the
+// state between substeps stays FP64.
+__device__ __forceinline__ double fp32_bg_angmom(double sbar, double vref, double R, double omega) {
+  return sbar * (vref + R * omega) * R;
+}
+
 #define TRAD_TPB 64
 #define NQ 5  // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Sigma
 #define NQ_MAX 6  // adiabatic: RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Sigma
@@ -524,6 +548,10 @@ struct trad_args {
   const double *dens, *vr, *vt;
   const double *energy;  // adiabatic decks (NQT = 6)
   double *out[NQ_MAX];
+  // FP32 bulk mode: the same fields stored in FP32 (fp32_store_bg)
+  const float *fdens, *fvr, *fvt;
+  float *fout[NQ];
+  const double *sbar, *vref;  // per-ring backgrounds of Sigma and vtheta
   double *vmed;  // ring means of vtheta for the azimuthal pass
   fg_dev_tables t;
   double dt, omega_frame;
@@ -592,6 +620,43 @@ struct ring_lanes {
     return m - r0;
   }
 };
+
+// The FP32 storage backgrounds of the rings a column strip marches over,
+// one ring per lane like ring_lanes (same first ring r0, reloaded with it)
+struct bg_lanes {
+  int r0;
+  double sb, vb;
+  __device__ __forceinline__ void load(const double *sbar, const double *vref, int base, int nr) {
+    r0 = base;
+    const int r = min(max(base + (int)threadIdx.x, 0), nr);
+    sb = sbar[r];
+    vb = vref[r];
+    asm volatile("" ::"v"(sb), "v"(vb));
+  }
+};
+
+// ring_mean_wave of a ring of vtheta stored as FP32 offsets from vb: the
+// same sequential sum of the FP64 values vb + offset
+__device__ void ring_mean_wave32(const float *v, double vb, int ns, double *out) {
+  __shared__ double buf[64 * RMW_CH];
+  const int lane = threadIdx.x;
+  const int nch = (ns + 64 * RMW_CH - 1) / (64 * RMW_CH);
+  float nx[RMW_CH];
+#pragma unroll
+  for (int u = 0; u < RMW_CH; u++) nx[u] = (lane + 64 * u < ns) ? v[lane + 64 * u] : 0.0f;
+  double sum = 0.0;
+  for (int c = 0; c < nch; c++) {
+#pragma unroll
+    for (int u = 0; u < RMW_CH; u++) buf[lane + 64 * u] = vb + (double)nx[u];
+    const int e0 = (c + 1) * 64 * RMW_CH + lane;
+#pragma unroll
+    for (int u = 0; u < RMW_CH; u++) nx[u] = (e0 + 64 * u < ns) ? v[e0 + 64 * u] : 0.0f;
+    __syncthreads();
+    if (lane == 0) sum = seq_sum_lds(buf, min(64 * RMW_CH, ns - c * 64 * RMW_CH), sum);
+    __syncthreads();
+  }
+  if (lane == 0) *out = sum / (double)ns;
+}
 
 // NQT transported quantities: 5, or 6 with the energy of an adiabatic deck
 // (TransportEuler.cu:60-79 sweeps it between the momenta and Sigma)
@@ -766,6 +831,10 @@ struct taz_args {
   const double *vmed;   // its ring means
   const double *q[NQ_MAX];  // quantities after the radial sweep (Sigma last)
   double *xd, *xvt, *xrmp, *xrmm, *xe;
+  // FP32 bulk mode: the same fields stored in FP32 (fp32_store_bg)
+  const float *fvt, *fq[NQ];
+  float *fxd, *fxvt, *fxrmp, *fxrmm;
+  const double *sbar, *vref;
   int *nshift;
   fg_dev_tables t;
   double dt, omega_frame;
@@ -953,13 +1022,39 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
     }
   }
   // every field's loads are issued before the first is staged, so the six
-  // fields cost one memory latency, not six
+  // fields cost one memory latency, not six.  FP32 mode: the fields are
+  // FP32 offsets from the ring's backgrounds (fp32_store_bg)
   double raw[NQT + 1][CL];
+  double sb = 0.0, vb = 0.0, lb = 0.0;
+  if constexpr (F32) {
+    float fr[NQT + 1][CL];
 #pragma unroll
-  for (int qq = 0; qq <= NQT; qq++) {
-    const double *src = qq < NQT ? a.q[qq] : a.vt;
+    for (int qq = 0; qq <= NQT; qq++) {
+      const float *src = qq < NQT ? a.fq[qq] : a.fvt;
 #pragma unroll
-    for (int m = 0; m < CL; m++) raw[qq][m] = src[base + jl[m]];
+      for (int m = 0; m < CL; m++) fr[qq][m] = src[base + jl[m]];
+    }
+    sb = a.sbar[i];
+    vb = a.vref[i];
+    const double Rm = t.Rmed[i];
+#pragma unroll
+    for (int m = 0; m < CL; m++) {
+      const double s = sb + (double)fr[4][m];
+      lb = fp32_bg_angmom(s, vb, Rm, a.omega_frame);  // the cell's angular momentum background
+      raw[0][m] = (double)fr[0][m];
+      raw[1][m] = (double)fr[1][m];
+      raw[2][m] = lb + (double)fr[2][m];
+      raw[3][m] = lb + (double)fr[3][m];
+      raw[4][m] = s;
+      raw[5][m] = vb + (double)fr[5][m];
+    }
+  } else {
+#pragma unroll
+    for (int qq = 0; qq <= NQT; qq++) {
+      const double *src = qq < NQT ? a.q[qq] : a.vt;
+#pragma unroll
+      for (int m = 0; m < CL; m++) raw[qq][m] = src[base + jl[m]];
+    }
   }
 #pragma unroll
   for (int qq = 0; qq <= NQT; qq++) {
@@ -1032,11 +1127,18 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
       int jo = g + sh;
       if (jo >= ns) jo -= ns;
       const long lo = base + jo;
-      a.xd[lo] = ostage[0][k];
-      a.xvt[lo] = ostage[1][k];
-      a.xrmp[lo] = ostage[2][k];
-      a.xrmm[lo] = ostage[3][k];
-      if constexpr (NQT == 6) a.xe[lo] = ostage[4][k];
+      if constexpr (F32) {
+        a.fxd[lo] = (float)(ostage[0][k] - sb);
+        a.fxvt[lo] = (float)(ostage[1][k] - vb);
+        a.fxrmp[lo] = (float)ostage[2][k];
+        a.fxrmm[lo] = (float)ostage[3][k];
+      } else {
+        a.xd[lo] = ostage[0][k];
+        a.xvt[lo] = ostage[1][k];
+        a.xrmp[lo] = ostage[2][k];
+        a.xrmm[lo] = ostage[3][k];
+        if constexpr (NQT == 6) a.xe[lo] = ostage[4][k];
+      }
     }
   }
 }
@@ -1085,6 +1187,12 @@ struct fin_args {
   double gamma;
   int adiabatic, cs_refresh;
   const double *sgacc;  // self-gravity: ring means of g_r for vtheta0 (else NULL)
+  // FP32 bulk mode: the fields of k_taz stored in FP32 (fp32_store_bg) with
+  // the backgrounds sbar (Sigma) and vmoy (vtheta, read before this pass
+  // overwrites it), and the next substep's Sigma background
+  const float *fxd, *fxvt, *fxrmp, *fxrmm;
+  const double *sbar;
+  double *sbar_next;
   double *vmoy, *newdt;
   double *fpart;  // FP32 mode: per-ring compensated star-force partials (4 per ring), else NULL
   int *crash;     // set to 1 when a stored density is negative or not finite
@@ -1181,7 +1289,7 @@ __device__ __forceinline__ void ring_min_store(double best, double *red, double 
 }
 
 #define FIN_OV_K 16  // cells per thread of the overlapped CFL pass (rings up to 3072 cells)
-template <int FIN_B, bool ADI = false>
+template <int FIN_B, bool ADI = false, bool S32 = false>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[4];
   __shared__ double s_vmoy;
@@ -1202,6 +1310,9 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const double inv3 = 1.0 / (Rm * Rm) / Rm;
   const bool radial = i > 0 && i < nr;  // vrad recovery reads rings i-1 and i
   bool bad = false;                      // a density below 0, infinite or NaN
+  // FP32 storage backgrounds of rings i-1, i, i+1 and of vtheta on ring i
+  const double sb_m = S32 && i > 0 ? a.sbar[i - 1] : 0.0, sb_i = S32 ? a.sbar[i] : 0.0;
+  const double vb_i = S32 ? a.vmoy[i] : 0.0;
   // FIN_B cells per thread per batch: all their loads first, then the math
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B];
@@ -1211,13 +1322,23 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       lrp[u] = lrm[u] = ld[u] = ldm[u] = lvt[u] = lp[u] = 0.0;
       if (j < ns) {
         const long l = base + j;
-        if (radial) {
-          lrp[u] = a.xrmp[l - ns];
-          lrm[u] = a.xrmm[l];
-          ldm[u] = a.xd[l - ns];
+        if constexpr (S32) {
+          if (radial) {
+            lrp[u] = (double)a.fxrmp[l - ns];
+            lrm[u] = (double)a.fxrmm[l];
+            ldm[u] = sb_m + (double)a.fxd[l - ns];
+          }
+          ld[u] = sb_i + (double)a.fxd[l];
+          lvt[u] = vb_i + (double)a.fxvt[l];
+        } else {
+          if (radial) {
+            lrp[u] = a.xrmp[l - ns];
+            lrm[u] = a.xrmm[l];
+            ldm[u] = a.xd[l - ns];
+          }
+          ld[u] = a.xd[l];
+          lvt[u] = a.xvt[l];
         }
-        ld[u] = a.xd[l];
-        lvt[u] = a.xvt[l];
         if (adi)
           lp[u] = a.xe[l];
         else if (a.temp)
@@ -1265,24 +1386,32 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   // at x = y = 0) from the densities just stored, summed by waves 1-3 while
   // lane 0 forms the ring mean; cells j = tid - 64 + 192 u per thread, one
   // compensated partial per thread, then per wave, merged in wave order
+  // (FP32 storage: the same loop sums the ring's Sigma for the next
+  // substep's background, in any order)
+  double ssum = 0.0;
   auto star_force = [&]() {
     for (int j = tid - 64; j < ns; j += 192) {
       const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
-      const double cm = surf * a.dens[base + j];
+      const double d = a.dens[base + j];
+      const double cm = surf * d;
       kadd(fsx, fcx, cm * dx * inv3);
       kadd(fsy, fcy, cm * dy * inv3);
+      if (S32) ssum += d;
     }
   };
   auto star_force_merge = [&]() {
-    __shared__ double fw[4][4];
+    __shared__ double fw[4][5];
     if (tid >= 64) {
       kwave(fsx, fcx);
       kwave(fsy, fcy);
+      if (S32)
+        for (int off = 32; off > 0; off >>= 1) ssum += __shfl_down(ssum, off);
       if ((tid & 63) == 0) {
         fw[tid >> 6][0] = fsx;
         fw[tid >> 6][1] = fcx;
         fw[tid >> 6][2] = fsy;
         fw[tid >> 6][3] = fcy;
+        fw[tid >> 6][4] = ssum;
       }
     }
     __syncthreads();
@@ -1296,6 +1425,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       a.fpart[4 * i + 1] = cx;
       a.fpart[4 * i + 2] = sy;
       a.fpart[4 * i + 3] = cy;
+      if (S32) a.sbar_next[i] = (fw[1][4] + fw[2][4] + fw[3][4]) / (double)ns;
     }
   };
   // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
@@ -1336,10 +1466,18 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
             pvr[u] = a.vr[l];
             if (adi) pcs[u] = a.cs[l];
             if (radial_n) {
-              prp[u] = a.xrmp[l];
-              prm[u] = a.xrmm[l + ns];
-              pd1[u] = a.xd[l + ns];
-              pd0[u] = a.xd[l];
+              if constexpr (S32) {
+                const double sb_p = a.sbar[i + 1];
+                prp[u] = (double)a.fxrmp[l];
+                prm[u] = (double)a.fxrmm[l + ns];
+                pd1[u] = sb_p + (double)a.fxd[l + ns];
+                pd0[u] = sb_i + (double)a.fxd[l];
+              } else {
+                prp[u] = a.xrmp[l];
+                prm[u] = a.xrmm[l + ns];
+                pd1[u] = a.xd[l + ns];
+                pd0[u] = a.xd[l];
+              }
             }
           }
         }
@@ -1408,10 +1546,18 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         pvr[u] = a.vr[l];  // written above by this thread
         if (adi) pcs[u] = a.cs[l];
         if (radial_n) {
-          prp[u] = a.xrmp[l];
-          prm[u] = a.xrmm[l + ns];
-          pd1[u] = a.xd[l + ns];
-          pd0[u] = a.xd[l];
+          if constexpr (S32) {
+            const double sb_p = a.sbar[i + 1];
+            prp[u] = (double)a.fxrmp[l];
+            prm[u] = (double)a.fxrmm[l + ns];
+            pd1[u] = sb_p + (double)a.fxd[l + ns];
+            pd0[u] = sb_i + (double)a.fxd[l];
+          } else {
+            prp[u] = a.xrmp[l];
+            prm[u] = a.xrmm[l + ns];
+            pd1[u] = a.xd[l + ns];
+            pd0[u] = a.xd[l];
+          }
         }
       }
     }
@@ -1444,13 +1590,14 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
 // contiguous band of rings, so neighbouring rings, which read each other's
 // momenta and densities, are processed side by side under one L2.
 #define FG_NXCD 8
-template <bool ADI>
+// ADI: adiabatic deck; S32: FP32 bulk mode (FP32 storage of k_taz's fields)
+template <bool ADI, bool S32 = false>
 __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   extern __shared__ double lds_fin[];
   const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
   const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
   if (i >= a.nr) return;
-  fin_ring<4, ADI>(a, i, lds_fin);
+  fin_ring<4, ADI, S32>(a, i, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
@@ -1481,7 +1628,9 @@ static size_t src32_lds_bytes(int th) { return (size_t)(6 * (th + HALO_R + HALO_
 struct src32_args {
   const double *dens, *vr, *vt;
   const double *vref;  // per-ring reference vtheta
-  double *odens, *ovr, *ovt, *opress;
+  const double *sbar;  // per-ring background of Sigma
+  float *odens, *ovr, *ovt;  // FP32 storage (fp32_store_bg)
+  double *opress;
   const double *nu;
   fg_dev_tables t;
   fg_planets_arg P;
@@ -1564,8 +1713,8 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
 #define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
   // per-ring tables of the tile's rows as in k_src, plus the reference
   // vtheta (0 outside the disk), and the potential's cos/sin per column
-  constexpr int RT_VREF = RT_N, MAXR = SRC32_MAX_TH + HALO_R + HALO_B;
-  __shared__ double rt[RT_N + 1][MAXR];
+  constexpr int RT_VREF = RT_N, RT_SBAR = RT_N + 1, MAXR = SRC32_MAX_TH + HALO_R + HALO_B;
+  __shared__ double rt[RT_N + 2][MAXR];
   __shared__ double ct[2][HC];
 
   // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
@@ -1597,11 +1746,13 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
         const double v2 = ring_table(t, a.nu, w + 8)[i];
         const int ez = t.ev_zone[i];
         const double vref = (gi >= 0 && gi < nr) ? a.vref[gi] : 0.0;
+        const double sbar = (gi >= 0 && gi < nr) ? a.sbar[gi] : 0.0;
         rt[w][r] = v0;
         rt[w + 4][r] = v1;
         rt[w + 8][r] = v2;
         if (w == 0) rt[RT_EVZ][r] = (double)ez;
         if (w == 1) rt[RT_VREF][r] = vref;
+        if (w == 2) rt[RT_SBAR][r] = sbar;
       }
       for (int c = tid; c < HC; c += SRC32_TPB) {
         const int j = GJ(c);
@@ -1784,10 +1935,10 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     const double *ev = evc[r];
     src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], ev, s, vr, vt);
     if (a.opress && i < nr) a.opress[l] = s * rt[RT_CS][r] * rt[RT_CS][r];
-    if (i == nr) {  // ghost ring
-      a.odens[l] = s;
-      a.ovr[l] = 0.0;
-      a.ovt[l] = 0.0;
+    if (i == nr) {  // ghost ring (never read)
+      a.odens[l] = 0.0f;
+      a.ovr[l] = 0.0f;
+      a.ovt[l] = 0.0f;
       continue;
     }
     double vrnew = 0.0;
@@ -1802,9 +1953,9 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
       vtnew = damp_toward(vtnew, ev[0], ev[2]);
       s = damp_toward(s, ev[0], rt[RT_SMED][r]);
     }
-    a.odens[l] = s;
-    a.ovr[l] = vrnew;
-    a.ovt[l] = vtnew;
+    a.odens[l] = (float)(s - rt[RT_SBAR][r]);
+    a.ovr[l] = (float)vrnew;
+    a.ovt[l] = (float)(vtnew - rt[RT_VREF][r]);
   }
 #undef GI
 #undef GJ
@@ -1826,7 +1977,7 @@ __global__ __launch_bounds__(SRC32_TPB) void k_src32(src32_args a) {
 __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   const int nr = a.nr, ns = a.ns;
   if (blk < nr) {
-    ring_mean_wave(a.vt + (long)blk * ns, ns, a.vmed + blk);
+    ring_mean_wave32(a.fvt + (long)blk * ns, a.vref[blk], ns, a.vmed + blk);
     return;
   }
   const int bid = blk - nr;
@@ -1845,28 +1996,31 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   const int m0 = max(b0 - 2, 0);
   ring_lanes rt;
   rt.load(t, m0 - 2, nr);
+  bg_lanes bg;
+  bg.load(a.sbar, a.vref, rt.r0, nr);
   // ring m's inputs are loaded one iteration ahead, unconditionally (as
-  // in k_trad)
-  double n_s, n_v0, n_v1, n_vtp, n_vt;
+  // in k_trad), as the stored FP32 offsets
+  float n_s, n_v0, n_v1, n_vtp, n_vt;
   {
     const long l = (long)m0 * ns + j;
-    n_s = a.dens[l];
-    n_v0 = a.vr[l];
-    n_v1 = a.vr[l + ns];
-    n_vtp = a.vt[(long)m0 * ns + jp];
-    n_vt = a.vt[l];
+    n_s = a.fdens[l];
+    n_v0 = a.fvr[l];
+    n_v1 = a.fvr[l + ns];
+    n_vtp = a.fvt[(long)m0 * ns + jp];
+    n_vt = a.fvt[l];
     asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
   }
   auto ring = [&](int m, auto store) {
     const int r = rt.at(t, m, nr);
-    const double c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
+    if (bg.r0 != rt.r0) bg.load(a.sbar, a.vref, rt.r0, nr);
+    const float c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
     {
       const long l = (long)min(m + 1, nr - 1) * ns;
-      n_s = a.dens[l + j];
+      n_s = a.fdens[l + j];
       n_v0 = c_v1;
-      n_v1 = a.vr[l + ns + j];
-      n_vtp = a.vt[l + jp];
-      n_vt = a.vt[l + j];
+      n_v1 = a.fvr[l + ns + j];
+      n_vtp = a.fvt[l + jp];
+      n_vt = a.fvt[l + j];
     }
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
@@ -1880,18 +2034,19 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     vr[1] = vr[2];
     const double Rm = lane_get(rt.Rmed, r);
     if (m < nr) {
-      const double s = c_s;
-      const double v0 = c_v0, v1 = c_v1;
+      const double vb = lane_get(bg.vb, r);
+      const double s = lane_get(bg.sb, r) + (double)c_s;
+      const double v0 = (double)c_v0, v1 = (double)c_v1;
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
-      Q[2][2] = s * (c_vtp + Rm * OF) * Rm;
-      Q[3][2] = s * (c_vt + Rm * OF) * Rm;
+      Q[2][2] = s * ((vb + (double)c_vtp) + Rm * OF) * Rm;
+      Q[3][2] = s * ((vb + (double)c_vt) + Rm * OF) * Rm;
       Q[4][2] = s;
       const float fs = (float)s;
       const float rs = rcp_f(fs + 1e-20f);
       for (int q = 0; q < NQ; q++) W[q][2] = (float)Q[q][2] * rs;
       sg[2] = fs;
-      vr[2] = (float)v0;
+      vr[2] = c_v0;
     } else {
       for (int q = 0; q < NQ; q++) {
         Q[q][2] = 0.0;
@@ -1927,10 +2082,21 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
       F[q][1] = cf * QS[q];
     }
     if (decltype(store)::value) {
+      // ring o = k - 1 is complete: stored as FP32 offsets from its backgrounds
       const int o = k - 1;
       const long lo = (long)o * ns + j;
       const float isurf = (float)lane_get(rt.isurf, r - 2);
-      for (int q = 0; q < NQ; q++) a.out[q][lo] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
+      const double sb = lane_get(bg.sb, r - 2);
+      double val[NQ];
+      for (int q = 0; q < NQ; q++) val[q] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
+      // Sigma as the reader will reconstruct it, the angular momenta's background
+      const float fs = (float)(val[4] - sb);
+      const double lb = fp32_bg_angmom(sb + (double)fs, lane_get(bg.vb, r - 2), lane_get(rt.Rmed, r - 2), OF);
+      a.fout[0][lo] = (float)val[0];
+      a.fout[1][lo] = (float)val[1];
+      a.fout[2][lo] = (float)(val[2] - lb);
+      a.fout[3][lo] = (float)(val[3] - lb);
+      a.fout[4][lo] = fs;
     }
   };
   int m = m0;
@@ -1998,7 +2164,9 @@ static int fused_pool_arrays(const fg_ctx *c) { return (c->p.adiabatic || c->p.s
 static fg_status fused_alloc(fg_ctx *c) {
   if (c->fu_ring) return FG_OK;
   const size_t S = (size_t)c->S;
-  const size_t fpart_bytes = 4 * (size_t)(c->nr + 1) * sizeof(double);
+  // per-ring star-force partials (4 per ring) and the two Sigma backgrounds
+  // of the FP32 storage
+  const size_t fpart_bytes = 6 * (size_t)(c->nr + 1) * sizeof(double);
   const size_t pool_bytes = (size_t)fused_pool_arrays(c) * S * sizeof(double);
   double *pool = nullptr, *fpart = nullptr, *h = nullptr, *d = nullptr;
   hipError_t e = hipMalloc((void **)&pool, pool_bytes);
@@ -2019,6 +2187,9 @@ static fg_status fused_alloc(fg_ctx *c) {
     return e == hipErrorOutOfMemory ? FG_E_NOMEM : FG_E_HIP;
   }
   c->fu_fpart = fpart;
+  c->fu_sbar[0] = fpart + 4 * (size_t)(c->nr + 1);
+  c->fu_sbar[1] = c->fu_sbar[0] + (c->nr + 1);
+  c->fu_sbar_cur = 0;
   c->h_fused = h;
   c->d_fused = d;
   c->fu_ring = pool;
@@ -2146,10 +2317,28 @@ __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const 
                            const double *dens, const double *energy, int *crash, fg_dev_tables t, int nr, int ns,
                            int fast);
 
+// ring means of a field (any summation order: the FP32 storage's Sigma
+// background of the first substep after a state was set)
+__global__ __launch_bounds__(256) void k_ring_bg(double *out, const double *f, int ns) {
+  __shared__ double w[4];
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < ns; j += 256) s += f[(long)i * ns + j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = (w[0] + w[1] + w[2] + w[3]) / (double)ns;
+}
+
 extern "C" fg_status fg_fused_prepare(fg_ctx *c, double DeltaT) {
   if (!c) return FG_E_INVALID;
   if (fg_status s = fused_supported(c)) return s;
   if (fg_status s = fused_alloc(c)) return s;
+  if (c->p.precision == FG_PREC_FP32) {
+    hipLaunchKernelGGL(k_ring_bg, dim3(c->nr), dim3(256), 0, c->stream, c->fu_sbar[c->fu_sbar_cur],
+                       (const double *)c->f[FG_DENS], c->ns);
+    FG_LAUNCH_CHECK();
+  }
   size_t shm = (size_t)(c->ns + 256) * sizeof(double);
   hipLaunchKernelGGL(k_cfl_ring, dim3(c->nr), dim3(256), shm, c->stream, c->Vmoy, c->newDT, c->f[FG_VRAD],
                      c->f[FG_VTHETA], c->f[FG_SOUNDSPEED], c->f[FG_DENS],
@@ -2276,9 +2465,10 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     s3.vr = sa.vr;
     s3.vt = sa.vt;
     s3.vref = c->Vmoy;  // ring means of vtheta from the previous substep
-    s3.odens = sa.odens;
-    s3.ovr = sa.ovr;
-    s3.ovt = sa.ovt;
+    s3.sbar = c->fu_sbar[c->fu_sbar_cur];
+    s3.odens = b.fBd();
+    s3.ovr = b.fBvr();
+    s3.ovt = b.fBvt();
     s3.opress = sa.opress;
     s3.nu = sa.nu;
     s3.t = sa.t;
@@ -2345,6 +2535,12 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.out[3] = b.Rtmm;
   ta.out[4] = adi ? b.Re : b.Rd;
   ta.out[5] = b.Rd;
+  ta.fdens = b.fBd();
+  ta.fvr = b.fBvr();
+  ta.fvt = b.fBvt();
+  for (int q = 0; q < NQ; q++) ta.fout[q] = b.fR(q);
+  ta.sbar = c->fu_sbar[c->fu_sbar_cur];
+  ta.vref = c->Vmoy;
   ta.t = c->t;
   ta.dt = dt;
   ta.omega_frame = omega_frame;
@@ -2377,6 +2573,14 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.xrmp = b.Xrmp;
   za.xrmm = b.Xrmm;
   za.xe = b.Xe;
+  za.fvt = b.fBvt();
+  for (int q = 0; q < NQ; q++) za.fq[q] = b.fR(q);
+  za.fxd = b.fXd();
+  za.fxvt = b.fXvt();
+  za.fxrmp = b.fXrmp();
+  za.fxrmm = b.fXrmm();
+  za.sbar = c->fu_sbar[c->fu_sbar_cur];
+  za.vref = c->Vmoy;
   za.nshift = c->Nshift;
   za.t = c->t;
   za.dt = dt;
@@ -2435,13 +2639,22 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.adiabatic = adi ? 1 : 0;
   fa.cs_refresh = (adi && (flags & FG_STEP_CLOSES_DT)) ? 1 : 0;
   fa.sgacc = sg ? c->sgacc_mean : nullptr;
+  fa.fxd = b.fXd();
+  fa.fxvt = b.fXvt();
+  fa.fxrmp = b.fXrmp();
+  fa.fxrmm = b.fXrmm();
+  fa.sbar = c->fu_sbar[c->fu_sbar_cur];
+  fa.sbar_next = c->fu_sbar[c->fu_sbar_cur ^ 1];
   prof_mark(c);
   const dim3 gfin(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD));
   if (adi)
     hipLaunchKernelGGL(k_fin<true>, gfin, dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
+  else if (fp32)
+    hipLaunchKernelGGL((k_fin<false, true>), gfin, dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   else
     hipLaunchKernelGGL(k_fin<false>, gfin, dim3(256), (size_t)ns * sizeof(double), c->stream, fa);
   FG_LAUNCH_CHECK();
+  if (fp32) c->fu_sbar_cur ^= 1;  // k_fin wrote the next substep's Sigma background
   prof_mark(c);
   if (c->nu_pending) {  // the refreshed viscosity is now the current one
     std::swap(c->nu, c->nu_next);
@@ -2540,7 +2753,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) fin_ring<1>(D[w.x].fin, w.y, lds_bfin);
+    if (D[w.x].active) fin_ring<1, false, true>(D[w.x].fin, w.y, lds_bfin);
     __syncthreads();
   }
 }
@@ -2749,10 +2962,16 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   s.dens = c->f[FG_DENS];
   s.vr = c->f[FG_VRAD];
   s.vt = c->f[FG_VTHETA];
+  // FP32 storage backgrounds: this substep reads fu_sbar[cur], its k_fin
+  // writes the other one for the next
+  const double *sbar = c->fu_sbar[c->fu_sbar_cur];
+  double *sbar_next = c->fu_sbar[c->fu_sbar_cur ^ 1];
+  c->fu_sbar_cur ^= 1;
   s.vref = c->Vmoy;
-  s.odens = fb.Bd;
-  s.ovr = fb.Bvr;
-  s.ovt = fb.Bvt;
+  s.sbar = sbar;
+  s.odens = fb.fBd();
+  s.ovr = fb.fBvr();
+  s.ovt = fb.fBvt();
   s.opress = nullptr;
   s.nu = c->nu;
   s.t = c->t;
@@ -2783,6 +3002,12 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   ta.out[2] = fb.Rtmp;
   ta.out[3] = fb.Rtmm;
   ta.out[4] = fb.Rd;
+  ta.fdens = fb.fBd();
+  ta.fvr = fb.fBvr();
+  ta.fvt = fb.fBvt();
+  for (int q = 0; q < NQ; q++) ta.fout[q] = fb.fR(q);
+  ta.sbar = sbar;
+  ta.vref = c->Vmoy;
   ta.vmed = c->VMed;
   ta.t = c->t;
   ta.dt = dt;
@@ -2803,6 +3028,14 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   za.xvt = fb.Xvt;
   za.xrmp = fb.Xrmp;
   za.xrmm = fb.Xrmm;
+  za.fvt = fb.fBvt();
+  for (int q = 0; q < NQ; q++) za.fq[q] = fb.fR(q);
+  za.fxd = fb.fXd();
+  za.fxvt = fb.fXvt();
+  za.fxrmp = fb.fXrmp();
+  za.fxrmm = fb.fXrmm();
+  za.sbar = sbar;
+  za.vref = c->Vmoy;
   za.nshift = c->Nshift;
   za.t = c->t;
   za.dt = dt;
@@ -2837,6 +3070,12 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fa.gamma = 0.0;
   fa.adiabatic = fa.cs_refresh = 0;
   fa.sgacc = nullptr;
+  fa.fxd = fb.fXd();
+  fa.fxvt = fb.fXvt();
+  fa.fxrmp = fb.fXrmp();
+  fa.fxrmm = fb.fXrmm();
+  fa.sbar = sbar;
+  fa.sbar_next = sbar_next;
   d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
   d.ncell = (long)nr * ns;
   d.B = red_blocks(d.ncell);

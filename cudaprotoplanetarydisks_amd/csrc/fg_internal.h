@@ -159,6 +159,11 @@ struct fg_ctx {
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results
   double *fu_fpart;   // FP32 mode: per-ring star-force partials (4 * NRAD)
+  // FP32 mode: per-ring background of Sigma the passes' FP32 offsets are
+  // taken from (ring means of the last state), double-buffered: fu_sbar_cur
+  // is read by this substep, the other written by its k_fin for the next
+  double *fu_sbar[2];
+  int fu_sbar_cur;
   // fused path's force and CFL count: written by the final reduction kernel
   // straight into coherent pinned host memory (no copy); device alias
   double *h_fused, *d_fused;
