@@ -171,10 +171,11 @@ __global__ __launch_bounds__(FFT_TPB) void k_fft_lines(fft_args a) {
   }
 }
 
-// k_fft_pow2 runs 1024 threads with 4 outputs each (FFT2_TPB * FFT2_U =
-// FFT_MAX_ELEMS): four complex outputs and their operands fit in few
-// registers, so several workgroups share a CU.
-#define FFT2_TPB 1024
+// k_fft_pow2 runs TPB threads with 4 outputs each (up to TPB * FFT2_U
+// elements per workgroup): four complex outputs and their operands fit in
+// few registers.  Rows of up to 2048 elements take 512 threads (up to four
+// workgroups per CU, one's loads and stores overlapping another's levels),
+// everything else 1024.
 #define FFT2_U 4
 
 // one radix-2 output, r = 0 with w0 then r = 1 with t1, accumulated from
@@ -206,8 +207,8 @@ __device__ __forceinline__ double2 bfly2(double2 x0, double2 x1, double2 w0, dou
 // same radix-2 expression of the same operands as when the levels run one
 // after the other through LDS, so both give the same bits (as does
 // k_fft_lines), with half the LDS round trips and barriers.
-template <int CB, int MODE>
-__global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
+template <int CB, int MODE, int TPB>
+__global__ __launch_bounds__(TPB) void k_fft_pow2(fft_args a) {
   extern __shared__ double2 buf[];
   constexpr int LCB = CB == 1 ? 0 : CB == 2 ? 1 : CB == 4 ? 2 : 3;
   const int N = a.n, tid = threadIdx.x;
@@ -217,9 +218,9 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
   if (a.elem_stride == 1) {
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
-      for (int o = tid; o < N; o += FFT2_TPB) buf[o * CB + cb] = fft_load<MODE>(a, g, line0 + cb, o);
+      for (int o = tid; o < N; o += TPB) buf[o * CB + cb] = fft_load<MODE>(a, g, line0 + cb, o);
   } else {
-    for (int e = tid; e < E; e += FFT2_TPB) buf[e] = fft_load<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB);
+    for (int e = tid; e < E; e += TPB) buf[e] = fft_load<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB);
   }
   __syncthreads();
   double2 out[FFT2_U];
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
     const double2 *w = a.tw + a.off[d];
 #pragma unroll
     for (int u = 0; u < FFT2_U; u++) {
-      const int e = tid + u * FFT2_TPB;
+      const int e = tid + u * TPB;
       out[u] = make_double2(0.0, 0.0);
       if (e < E) {
         const int cb = e & (CB - 1), o = e >> LCB;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < FFT2_U; u++) {
-      const int e = tid + u * FFT2_TPB;
+      const int e = tid + u * TPB;
       if (e < E) buf[e] = out[u];
     }
     __syncthreads();
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
     const double2 w0 = w[0];
 #pragma unroll
     for (int u = 0; u < FFT2_U; u++) {
-      const int e = tid + u * FFT2_TPB;
+      const int e = tid + u * TPB;
       if (e < E) {
         const int cb = e & (CB - 1), o = e >> LCB;
         const int s = o & (S - 1), kk = o >> sh;
@@ -278,14 +279,14 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < FFT2_U; u++) {
-      const int e = tid + u * FFT2_TPB;
+      const int e = tid + u * TPB;
       if (e < E) buf[e] = out[u];
     }
     __syncthreads();
     d--;
   }
   // pairs of levels: lower d (L/2, 2S), upper d - 1 (L, S); one group of
-  // four per thread (E / 4 <= FFT2_TPB)
+  // four per thread (E / 4 <= TPB)
   for (; d >= 1; d -= 2) {
     const int L = a.L[d - 1], sh = a.sh[d - 1], q = L >> 2;  // q = L/4
     const int S = 1 << sh;
@@ -322,9 +323,9 @@ __global__ __launch_bounds__(FFT2_TPB) void k_fft_pow2(fft_args a) {
   if (a.elem_stride == 1) {
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
-      for (int o = tid; o < N; o += FFT2_TPB) fft_store<MODE>(a, g, line0 + cb, o, buf[o * CB + cb]);
+      for (int o = tid; o < N; o += TPB) fft_store<MODE>(a, g, line0 + cb, o, buf[o * CB + cb]);
   } else {
-    for (int e = tid; e < E; e += FFT2_TPB) fft_store<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB, buf[e]);
+    for (int e = tid; e < E; e += TPB) fft_store<MODE>(a, g, line0 + (e & (CB - 1)), e >> LCB, buf[e]);
   }
 }
 
@@ -462,16 +463,25 @@ static void fft_launch(fg_ctx *c, const fft_args &a, const fg_fft_plan *pl, int 
   if (a.elem_stride != 1) blocks = 8 * ((blocks + 7) / 8);  // column groups in XCD order (fft_block_line)
   const size_t shm = (size_t)a.n * a.lines_per_block * sizeof(double2);
   const dim3 grid(blocks, ngrids);
+  const bool wide = a.n * a.lines_per_block > 512 * FFT2_U;  // more than 2048 elements: 1024 threads
   if (!pl->pow2s)
     hipLaunchKernelGGL(k_fft_lines<MODE>, grid, dim3(FFT_TPB), shm, c->stream, a);
+  else if (wide && a.lines_per_block == 8)
+    hipLaunchKernelGGL((k_fft_pow2<8, MODE, 1024>), grid, dim3(1024), shm, c->stream, a);
+  else if (wide && a.lines_per_block == 4)
+    hipLaunchKernelGGL((k_fft_pow2<4, MODE, 1024>), grid, dim3(1024), shm, c->stream, a);
+  else if (wide && a.lines_per_block == 2)
+    hipLaunchKernelGGL((k_fft_pow2<2, MODE, 1024>), grid, dim3(1024), shm, c->stream, a);
+  else if (wide)
+    hipLaunchKernelGGL((k_fft_pow2<1, MODE, 1024>), grid, dim3(1024), shm, c->stream, a);
   else if (a.lines_per_block == 8)
-    hipLaunchKernelGGL((k_fft_pow2<8, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+    hipLaunchKernelGGL((k_fft_pow2<8, MODE, 512>), grid, dim3(512), shm, c->stream, a);
   else if (a.lines_per_block == 4)
-    hipLaunchKernelGGL((k_fft_pow2<4, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+    hipLaunchKernelGGL((k_fft_pow2<4, MODE, 512>), grid, dim3(512), shm, c->stream, a);
   else if (a.lines_per_block == 2)
-    hipLaunchKernelGGL((k_fft_pow2<2, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+    hipLaunchKernelGGL((k_fft_pow2<2, MODE, 512>), grid, dim3(512), shm, c->stream, a);
   else
-    hipLaunchKernelGGL((k_fft_pow2<1, MODE>), grid, dim3(FFT2_TPB), shm, c->stream, a);
+    hipLaunchKernelGGL((k_fft_pow2<1, MODE, 512>), grid, dim3(512), shm, c->stream, a);
 }
 
 static fg_status fft_pass(fg_ctx *c, int pass, int inverse, int mode, double *g0, double *g1, double dt = 0.0,
@@ -501,9 +511,13 @@ static fg_status fft_pass(fg_ctx *c, int pass, int inverse, int mode, double *g0
   a.kern[1] = (const double2 *)c->sg_Krc;
   a.out[0] = (double2 *)out0;
   a.out[1] = (double2 *)(out1 ? out1 : out0);
-  // lines staged per workgroup: up to FFT_MAX_ELEMS elements, a power of
-  // two up to 8 for k_fft_pow2 (rows 2 at a time, columns 4 at NSEC 1536)
-  int cb = FFT_MAX_ELEMS / pl->n;
+  // lines staged per workgroup: k_fft_lines up to FFT_MAX_ELEMS elements;
+  // k_fft_pow2 rows: up to 2 lines of at most 2048 elements together (512
+  // threads), a longer line alone (1024 threads); columns: a power of two
+  // up to 8 lines of at most FFT_MAX_ELEMS elements (1024 threads; four
+  // columns give 64-byte pieces of each row, two only 32: measured 53 vs
+  // 59 us for the forward column pass at 1024 x 1536)
+  int cb = ((pl->pow2s && pass == 0) ? 512 * FFT2_U : FFT_MAX_ELEMS) / pl->n;
   cb = cb >= 8 ? 8 : cb >= 4 ? 4 : cb >= 2 ? 2 : 1;
   if (pass == 0) {  // rows: contiguous
     a.nlines = mode == FFT_FILL ? c->nr : n1;
