@@ -381,15 +381,15 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--n-train", type=int, default=512)
     t.add_argument("--nrad", type=int, default=128)
-    t.add_argument("--orbits", type=float, default=150.0)
-    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--orbits", type=float, default=300.0)
+    t.add_argument("--steps", type=int, default=8000)
     t.add_argument("--seed", type=int, default=20261202)
     t.add_argument("--out", default=WEIGHTS)
     w = sub.add_parser("warmstart")
     w.add_argument("--weights", default=WEIGHTS)
-    w.add_argument("--n-eval", type=int, default=16)
-    w.add_argument("--max-orbits", type=int, default=100)
-    w.add_argument("--eps", type=float, nargs="+", default=[2e-3, 1e-3, 5e-4, 2e-4])
+    w.add_argument("--n-eval", type=int, default=32)
+    w.add_argument("--max-orbits", type=int, default=300)
+    w.add_argument("--eps", type=float, nargs="+", default=[1e-3, 5e-4, 2e-4, 1e-4, 5e-5])
     s = sub.add_parser("study")
     s.add_argument("--n-train", type=int, default=512)
     s.add_argument("--n-eval", type=int, default=32)
