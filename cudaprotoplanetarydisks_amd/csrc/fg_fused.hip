@@ -1313,16 +1313,19 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   // FP32 storage backgrounds of rings i-1, i, i+1 and of vtheta on ring i
   const double sb_m = S32 && i > 0 ? a.sbar[i - 1] : 0.0, sb_i = S32 ? a.sbar[i] : 0.0;
   const double vb_i = S32 ? a.vmoy[i] : 0.0;
+  double vsum = 0.0, ssum = 0.0;  // FP32 mode: this thread's sums of vtheta and Sigma
   // FIN_B cells per thread per batch: all their loads first, then the math
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
-    double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B];
+    double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B], lcs[FIN_B], lsn[FIN_B];
 #pragma unroll
     for (int u = 0; u < FIN_B; u++) {
       const int j = j0 + u * 256;
-      lrp[u] = lrm[u] = ld[u] = ldm[u] = lvt[u] = lp[u] = 0.0;
+      lrp[u] = lrm[u] = ld[u] = ldm[u] = lvt[u] = lp[u] = lcs[u] = lsn[u] = 0.0;
       if (j < ns) {
         const long l = base + j;
         if constexpr (S32) {
+          lcs[u] = t.cos_cell[j];
+          lsn[u] = t.sin_cell[j];
           if (radial) {
             lrp[u] = (double)a.fxrmp[l - ns];
             lrm[u] = (double)a.fxrmm[l];
@@ -1362,6 +1365,13 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     a.vr[l] = vr;
     a.vt[l] = vt;
     ring[j] = vt;
+    if constexpr (S32) {
+      vsum += vt;
+      ssum += s;
+      const double cm = surf * s;
+      kadd(fsx, fcx, cm * (Rm * lcs[u]) * inv3);
+      kadd(fsy, fcy, cm * (Rm * lsn[u]) * inv3);
+    }
     if (adi) {
       double e = lp[u];
       if (damp_i) e = damp_toward(e, lam_i, t.EnergyMed[i]);
@@ -1382,52 +1392,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   }
   if (bad) *a.crash = 1;
   __syncthreads();
-  // FP32 mode: gravity of the ring's cells on the star (Kernels.cu:703-727
-  // at x = y = 0) from the densities just stored, summed by waves 1-3 while
-  // lane 0 forms the ring mean; cells j = tid - 64 + 192 u per thread, one
-  // compensated partial per thread, then per wave, merged in wave order
-  // (FP32 storage: the same loop sums the ring's Sigma for the next
-  // substep's background, in any order)
-  double ssum = 0.0;
-  auto star_force = [&]() {
-    for (int j = tid - 64; j < ns; j += 192) {
-      const double dx = Rm * t.cos_cell[j], dy = Rm * t.sin_cell[j];
-      const double d = a.dens[base + j];
-      const double cm = surf * d;
-      kadd(fsx, fcx, cm * dx * inv3);
-      kadd(fsy, fcy, cm * dy * inv3);
-      if (S32) ssum += d;
-    }
-  };
-  auto star_force_merge = [&]() {
-    __shared__ double fw[4][5];
-    if (tid >= 64) {
-      kwave(fsx, fcx);
-      kwave(fsy, fcy);
-      if (S32)
-        for (int off = 32; off > 0; off >>= 1) ssum += __shfl_down(ssum, off);
-      if ((tid & 63) == 0) {
-        fw[tid >> 6][0] = fsx;
-        fw[tid >> 6][1] = fcx;
-        fw[tid >> 6][2] = fsy;
-        fw[tid >> 6][3] = fcy;
-        fw[tid >> 6][4] = ssum;
-      }
-    }
-    __syncthreads();
-    if (tid == 64) {
-      double sx = fw[1][0], cx = fw[1][1], sy = fw[1][2], cy = fw[1][3];
-      for (int w = 2; w < 4; w++) {
-        kmerge(sx, cx, fw[w][0], fw[w][1]);
-        kmerge(sy, cy, fw[w][2], fw[w][3]);
-      }
-      a.fpart[4 * i] = sx;
-      a.fpart[4 * i + 1] = cx;
-      a.fpart[4 * i + 2] = sy;
-      a.fpart[4 * i + 3] = cy;
-      if (S32) a.sbar_next[i] = (fw[1][4] + fw[2][4] + fw[3][4]) / (double)ns;
-    }
-  };
   // ConditionCFLKernel2D1 + 2D2 for ring i (Kernels.cu:1284-1329)
   const double dxrad = t.Rsup[i] - t.Rinf[i];
   const double dxtheta = t.dxtheta_cfl[i];
@@ -1435,6 +1399,48 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const double cs = t.cs_iso[i];
   const bool radial_n = i + 1 < nr;  // vrad of ring i+1 (0 on the ghost ring)
   double best = 1e30;
+  double vm;
+  if constexpr (S32) {
+    // FP32 mode, which does not replay the reference's summation order: the
+    // ring mean of vtheta, the gravity of the ring's cells on the star
+    // (Kernels.cu:703-727 at x = y = 0: the star sits at the origin with no
+    // smoothing, so 1/d^3 is one value per ring) and the ring's Sigma sum
+    // (the next substep's background) come from the per-thread partials of
+    // the pass above, reduced per wave by a shuffle tree and over the waves
+    // in order; the force as one compensated partial per ring
+    __shared__ double fw[4][6];
+    kwave(fsx, fcx);
+    kwave(fsy, fcy);
+    for (int off = 32; off > 0; off >>= 1) {
+      vsum += __shfl_down(vsum, off);
+      ssum += __shfl_down(ssum, off);
+    }
+    if ((tid & 63) == 0) {
+      double *w = fw[tid >> 6];
+      w[0] = fsx;
+      w[1] = fcx;
+      w[2] = fsy;
+      w[3] = fcy;
+      w[4] = vsum;
+      w[5] = ssum;
+    }
+    __syncthreads();
+    vm = (((fw[0][4] + fw[1][4]) + fw[2][4]) + fw[3][4]) / (double)ns;
+    if (tid == 0) {
+      double sx = fw[0][0], cx = fw[0][1], sy = fw[0][2], cy = fw[0][3];
+      for (int w = 1; w < 4; w++) {
+        kmerge(sx, cx, fw[w][0], fw[w][1]);
+        kmerge(sy, cy, fw[w][2], fw[w][3]);
+      }
+      a.fpart[4 * i] = sx;
+      a.fpart[4 * i + 1] = cx;
+      a.fpart[4 * i + 2] = sy;
+      a.fpart[4 * i + 3] = cy;
+      a.sbar_next[i] = (((fw[0][5] + fw[1][5]) + fw[2][5]) + fw[3][5]) / (double)ns;
+      a.vmoy[i] = vm;
+    }
+    if (i == 0) return;
+  } else {
   // Rings of up to FIN_OV_K * 192 cells: lane 0 of wave 0 forms the ring
   // mean while waves 1-3 load and evaluate the parts of each cell's time
   // step that do not depend on it, the radial, sound-speed and shear terms
@@ -1448,8 +1454,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
       m /= (double)ns;
       a.vmoy[i] = m;
       s_vmoy = m;
-    } else if (tid >= 64) {
-      if (a.fpart) star_force();
     }
     if (tid >= 64 && i > 0) {
       const double invdt1r = cs / mindx;
@@ -1503,7 +1507,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         }
       }
     }
-    if (a.fpart) star_force_merge();
     __syncthreads();
     if (i == 0) return;
     if (tid >= 64) {
@@ -1527,13 +1530,11 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     m /= (double)ns;
     a.vmoy[i] = m;
     s_vmoy = m;
-  } else if (tid >= 64 && a.fpart) {
-    star_force();
   }
-  if (a.fpart) star_force_merge();
   __syncthreads();
   if (i == 0) return;
-  const double vm = s_vmoy;
+  vm = s_vmoy;
+  }
   const recip64 rdr(dxrad), rdt(dxtheta);
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B], pcs[FIN_B];
@@ -1597,7 +1598,7 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
   const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
   if (i >= a.nr) return;
-  fin_ring<4, ADI, S32>(a, i, lds_fin);
+  fin_ring<S32 ? 2 : 4, ADI, S32>(a, i, lds_fin);
 }
 
 // ================================================== FP32 bulk mode ====
