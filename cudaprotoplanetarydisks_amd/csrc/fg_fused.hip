@@ -29,6 +29,9 @@
 // adiabatic, with or without self-gravity (FP32 bulk mode: isothermal
 // without self-gravity).  Everything else uses the per-routine path.
 #include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -536,6 +539,17 @@ __global__ __launch_bounds__(256) void k_src3(src3_args a) {
 // fields (a few 1e-3 of vtheta, the non-axisymmetric part of Sigma), so
 // their FP32 rounding stays far below the FP32 arithmetic's own error; the
 // state between substeps stays FP64.
+// Near the planets, where circumplanetary material stays for many orbits
+// and the rounding of the stored Sigma would add up there, the rings
+// [lob0, lob1) store Sigma as a float pair: the high part where every ring
+// stores its float (index l < S of an array of S doubles) and the rounding
+// residual at S + l, in the array's otherwise unused upper half; a reader
+// adds the residual only on those rings.  (Pairs for the velocities and
+// momenta as well change nothing measurable.)  The upper-half slots of the
+// ghost ring NRAD are never written and stay zero: loads that must be
+// issued on every iteration take them for rings outside the band.
+__device__ __forceinline__ bool in_lob(int i, int b0, int b1) { return i >= b0 && i < b1; }
+__device__ __forceinline__ double rec32(double bg, float hi, float lo) { return bg + ((double)hi + (double)lo); }
 __device__ __forceinline__ double fp32_bg_angmom(double sbar, double vref, double R, double omega) {
   return sbar * (vref + R * omega) * R;
 }
@@ -556,7 +570,8 @@ struct trad_args {
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, nstrips;
-  int band;  // rings per column strip
+  int band;        // rings per column strip
+  int lob0, lob1;  // FP32 storage: rings stored as float pairs
 };
 
 // ComputeAverageThetaVelocitiesKernel (Kernels.cu:1070-1081) for one ring
@@ -839,6 +854,7 @@ struct taz_args {
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, fast;
+  int lob0, lob1;  // FP32 storage: rings stored as float pairs
 };
 
 
@@ -990,13 +1006,19 @@ __device__ __forceinline__ void wave_sweep32(double (&q)[NQ][CL], const float (&
 
 // chunk `chunk` of ring i, one wave; F32 selects the FP32 bulk-mode sweeps;
 // NQT = 6 adds the energy of an adiabatic deck (FP64 only)
-template <int CL, bool F32, int NQT = 5>
-__device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
+// LDS of one chunk: the load stage, then the output stages (row 0 reused)
+template <int CL, int NQT>
+struct tazw_lds {
+  double ostage[NQT == 6 ? 5 : 4][64 * CL];
+};
+// PAIRS: a ring of the FP32 mode stored as float pairs (its own instance,
+// so that the other rings run without the residuals' loads and branches)
+template <int CL, bool F32, int NQT = 5, bool PAIRS = false>
+__device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk, tazw_lds<CL, NQT> &lds) {
   constexpr int N = 64 * CL, P = N - TAZW_HL - TAZW_HR;
-  constexpr int NOUT = NQT == 6 ? 5 : 4;
   static_assert(NQT == 5 || !F32, "the FP32 sweeps carry five quantities");
-  // LDS: the load stage, then the output stages (row 0 reused)
-  __shared__ double ostage[NOUT][N];
+  static_assert(F32 || !PAIRS, "float pairs are an FP32 storage form");
+  auto &ostage = lds.ostage;
   double *stage = ostage[0];
   const int ns = a.ns, l = threadIdx.x;
   const int g0 = chunk * P - TAZW_HL;  // unwrapped index of the chunk's first cell
@@ -1026,20 +1048,24 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
   // FP32 offsets from the ring's backgrounds (fp32_store_bg)
   double raw[NQT + 1][CL];
   double sb = 0.0, vb = 0.0, lb = 0.0;
+  constexpr bool lob = PAIRS;  // FP32 storage: this ring is stored as float pairs
+  const long S = (long)(a.nr + 1) * ns;
   if constexpr (F32) {
-    float fr[NQT + 1][CL];
+    float fr[NQT + 1][CL], fsl[CL];
 #pragma unroll
     for (int qq = 0; qq <= NQT; qq++) {
       const float *src = qq < NQT ? a.fq[qq] : a.fvt;
 #pragma unroll
       for (int m = 0; m < CL; m++) fr[qq][m] = src[base + jl[m]];
     }
+#pragma unroll
+    for (int m = 0; m < CL; m++) fsl[m] = lob ? a.fq[4][S + base + jl[m]] : 0.f;
     sb = a.sbar[i];
     vb = a.vref[i];
     const double Rm = t.Rmed[i];
 #pragma unroll
     for (int m = 0; m < CL; m++) {
-      const double s = sb + (double)fr[4][m];
+      const double s = rec32(sb, fr[4][m], fsl[m]);
       lb = fp32_bg_angmom(s, vb, Rm, a.omega_frame);  // the cell's angular momentum background
       raw[0][m] = (double)fr[0][m];
       raw[1][m] = (double)fr[1][m];
@@ -1128,10 +1154,13 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
       if (jo >= ns) jo -= ns;
       const long lo = base + jo;
       if constexpr (F32) {
-        a.fxd[lo] = (float)(ostage[0][k] - sb);
+        const double xd = ostage[0][k] - sb;
+        const float hd = (float)xd;
+        a.fxd[lo] = hd;
         a.fxvt[lo] = (float)(ostage[1][k] - vb);
         a.fxrmp[lo] = (float)ostage[2][k];
         a.fxrmm[lo] = (float)ostage[3][k];
+        if constexpr (lob) a.fxd[S + lo] = (float)(xd - (double)hd);
       } else {
         a.xd[lo] = ostage[0][k];
         a.xvt[lo] = ostage[1][k];
@@ -1146,7 +1175,11 @@ __device__ __forceinline__ void tazw_body(const taz_args &a, int i, int chunk) {
 // blockIdx.x = chunk, blockIdx.y = ring
 template <int CL, bool F32, int NQT = 5>
 __global__ __launch_bounds__(64) void k_tazw(taz_args a) {
-  tazw_body<CL, F32, NQT>(a, blockIdx.y, blockIdx.x);
+  __shared__ tazw_lds<CL, NQT> sh;
+  if (F32 && in_lob(blockIdx.y, a.lob0, a.lob1))
+    tazw_body<CL, F32, NQT, F32>(a, blockIdx.y, blockIdx.x, sh);
+  else
+    tazw_body<CL, F32, NQT, false>(a, blockIdx.y, blockIdx.x, sh);
 }
 
 // cells per lane of the wave kernel for a ring of ns cells: CL = 4 or 5
@@ -1200,6 +1233,7 @@ struct fin_args {
   fg_dev_tables t;
   double dt, omega_frame;
   int nr, ns, evanescent, fast;
+  int lob0, lob1;  // FP32 storage: rings stored as float pairs
 };
 
 
@@ -1257,13 +1291,70 @@ __device__ __forceinline__ void force_rings_final(const double *fpart, int nr, d
 }
 
 // FP32 mode: the star force from the per-ring partials, then the CFL count
-// (ConditionCFLKernel2D2 + 2D3, Kernels.cu:1331-1359) in the same launch
+// (ConditionCFLKernel2D2 + 2D3, Kernels.cu:1331-1359) in the same launch.
+// Up to 1024 rings a thread's force partials (rings tid * per + q, merged
+// in q order as force_rings_final does) are loaded together with its CFL
+// inputs, so the block waits for memory once.
+// The launch's sequence number goes last into slot FG_FUSED_SEQ_SLOT of the
+// result block out (behind a system-scope fence: every result above is
+// visible to the host first; thread 0 and the lanes that wrote them are in
+// wave 0).
+__device__ __forceinline__ void seq_store(double *out, double seq) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    out[FG_FUSED_SEQ_SLOT] = seq;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_force_rings_cfl(const double *fpart, int nr, double *out2, double *cfl,
                                                          double *dt1d, const double *vmoy, const double *newdt,
-                                                         const double *invRmed, double coef, double DeltaT) {
-  force_rings_final(fpart, nr, out2);
-  __syncthreads();
-  cfl_final(cfl, dt1d, vmoy, newdt, invRmed, coef, DeltaT, nr);
+                                                         const double *invRmed, double coef, double DeltaT,
+                                                         double seq) {
+  if (nr > 4 * 256) {
+    force_rings_final(fpart, nr, out2);
+    __syncthreads();
+    cfl_final(cfl, dt1d, vmoy, newdt, invRmed, coef, DeltaT, nr);
+    seq_store(out2, seq);
+    return;
+  }
+  __shared__ double ws[4][4];
+  const int tid = threadIdx.x, per = (nr + 255) / 256;
+  double fp[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = tid * per + q;
+#pragma unroll
+    for (int c = 0; c < 4; c++) fp[q][c] = (q < per && i < nr) ? fpart[4 * i + c] : 0.0;
+  }
+  const double best = cfl_ring_min(dt1d, vmoy, newdt, invRmed, coef, nr, tid, 256);
+  double sx = 0.0, cx = 0.0, sy = 0.0, cy = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (q < per && tid * per + q < nr) {
+      kmerge(sx, cx, fp[q][0], fp[q][1]);
+      kmerge(sy, cy, fp[q][2], fp[q][3]);
+    }
+  }
+  kwave(sx, cx);
+  kwave(sy, cy);
+  if ((tid & 63) == 0) {
+    ws[tid >> 6][0] = sx;
+    ws[tid >> 6][1] = cx;
+    ws[tid >> 6][2] = sy;
+    ws[tid >> 6][3] = cy;
+  }
+  const double m = block_min256(best);  // (its barrier also orders ws)
+  if (tid == 0) {
+    double a = ws[0][0], b = ws[0][1], c = ws[0][2], d = ws[0][3];
+    for (int w = 1; w < 4; w++) {
+      kmerge(a, b, ws[w][0], ws[w][1]);
+      kmerge(c, d, ws[w][2], ws[w][3]);
+    }
+    out2[0] = a + b;
+    out2[1] = c + d;
+  }
+  cfl_count_store(cfl, m, DeltaT);
+  seq_store(out2, seq);
 }
 
 // one ring; ring holds NSEC doubles (the ring's final vtheta).  FIN_B
@@ -1289,7 +1380,9 @@ __device__ __forceinline__ void ring_min_store(double best, double *red, double 
 }
 
 #define FIN_OV_K 16  // cells per thread of the overlapped CFL pass (rings up to 3072 cells)
-template <int FIN_B, bool ADI = false, bool S32 = false>
+// PAIRS: FP32 mode, ring i - 1, i or i + 1 stored as float pairs (its own
+// instance, so that the other rings run without the residuals' loads)
+template <int FIN_B, bool ADI = false, bool S32 = false, bool PAIRS = false>
 __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring) {
   __shared__ double red[4];
   __shared__ double s_vmoy;
@@ -1314,6 +1407,10 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   const double sb_m = S32 && i > 0 ? a.sbar[i - 1] : 0.0, sb_i = S32 ? a.sbar[i] : 0.0;
   const double vb_i = S32 ? a.vmoy[i] : 0.0;
   double vsum = 0.0, ssum = 0.0;  // FP32 mode: this thread's sums of vtheta and Sigma
+  // FP32 storage: Sigma of rings i - 1, i and i + 1 stored as float pairs
+  const bool lob_m = PAIRS && in_lob(i - 1, a.lob0, a.lob1), lob_i = PAIRS && in_lob(i, a.lob0, a.lob1);
+  const bool lob_p = PAIRS && in_lob(i + 1, a.lob0, a.lob1);
+  const long S2 = (long)(nr + 1) * ns;
   // FIN_B cells per thread per batch: all their loads first, then the math
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double lrp[FIN_B], lrm[FIN_B], ld[FIN_B], ldm[FIN_B], lvt[FIN_B], lp[FIN_B], lcs[FIN_B], lsn[FIN_B];
@@ -1326,13 +1423,18 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
         if constexpr (S32) {
           lcs[u] = t.cos_cell[j];
           lsn[u] = t.sin_cell[j];
+          float hdm = 0.f, ydm = 0.f, yd = 0.f;
           if (radial) {
             lrp[u] = (double)a.fxrmp[l - ns];
             lrm[u] = (double)a.fxrmm[l];
-            ldm[u] = sb_m + (double)a.fxd[l - ns];
+            hdm = a.fxd[l - ns];
           }
-          ld[u] = sb_i + (double)a.fxd[l];
+          const float hd = a.fxd[l];
           lvt[u] = vb_i + (double)a.fxvt[l];
+          if (radial && lob_m) ydm = a.fxd[S2 + l - ns];  // Sigma residuals of float-pair rings
+          if (lob_i) yd = a.fxd[S2 + l];
+          if (radial) ldm[u] = rec32(sb_m, hdm, ydm);
+          ld[u] = rec32(sb_i, hd, yd);
         } else {
           if (radial) {
             lrp[u] = a.xrmp[l - ns];
@@ -1470,18 +1572,10 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
             pvr[u] = a.vr[l];
             if (adi) pcs[u] = a.cs[l];
             if (radial_n) {
-              if constexpr (S32) {
-                const double sb_p = a.sbar[i + 1];
-                prp[u] = (double)a.fxrmp[l];
-                prm[u] = (double)a.fxrmm[l + ns];
-                pd1[u] = sb_p + (double)a.fxd[l + ns];
-                pd0[u] = sb_i + (double)a.fxd[l];
-              } else {
-                prp[u] = a.xrmp[l];
-                prm[u] = a.xrmm[l + ns];
-                pd1[u] = a.xd[l + ns];
-                pd0[u] = a.xd[l];
-              }
+              prp[u] = a.xrmp[l];
+              prm[u] = a.xrmm[l + ns];
+              pd1[u] = a.xd[l + ns];
+              pd0[u] = a.xd[l];
             }
           }
         }
@@ -1551,8 +1645,12 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
             const double sb_p = a.sbar[i + 1];
             prp[u] = (double)a.fxrmp[l];
             prm[u] = (double)a.fxrmm[l + ns];
-            pd1[u] = sb_p + (double)a.fxd[l + ns];
-            pd0[u] = sb_i + (double)a.fxd[l];
+            const float hd1 = a.fxd[l + ns], hd0 = a.fxd[l];
+            float yd1 = 0.f, yd0 = 0.f;
+            if (lob_i) yd0 = a.fxd[S2 + l];  // Sigma residuals of float-pair rings
+            if (lob_p) yd1 = a.fxd[S2 + l + ns];
+            pd1[u] = rec32(sb_p, hd1, yd1);
+            pd0[u] = rec32(sb_i, hd0, yd0);
           } else {
             prp[u] = a.xrmp[l];
             prm[u] = a.xrmm[l + ns];
@@ -1598,7 +1696,14 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
   const int band = (a.nr + FG_NXCD - 1) / FG_NXCD;
   const int i = (int)(blockIdx.x % FG_NXCD) * band + (int)(blockIdx.x / FG_NXCD);
   if (i >= a.nr) return;
-  fin_ring<S32 ? 2 : 4, ADI, S32>(a, i, lds_fin);
+  if constexpr (S32) {
+    if (in_lob(i, a.lob0 - 1, a.lob1 + 1))
+      fin_ring<2, ADI, true, true>(a, i, lds_fin);
+    else
+      fin_ring<2, ADI, true, false>(a, i, lds_fin);
+  } else {
+    fin_ring<4, ADI, false>(a, i, lds_fin);
+  }
 }
 
 // ================================================== FP32 bulk mode ====
@@ -1637,6 +1742,7 @@ struct src32_args {
   fg_planets_arg P;
   double dt, omega_frame, domega;
   int nr, ns, indirect, evanescent, corotating;
+  int lob0, lob1;  // FP32 storage: rings stored as float pairs
 };
 
 // planet part of the potential at one cell (the star's -1/r is applied per
@@ -1673,18 +1779,24 @@ __device__ __forceinline__ void src32_adjust(const src32_args &a, int i, double 
 
 #define SRC32_OWN_MAX (SRC32_MAX_TH * TW / SRC32_TPB)  // own cells per thread
 
-// FP64 state of own cell q of the tile (0 outside the grid)
-__device__ __forceinline__ void own_load(const src32_args &a, int i0, int j0, int th, int q, double &s, double &vr,
-                                         double &vt) {
-  s = vr = vt = 0.0;
-  if (q < th * TW) {
-    const int i = i0 + q / TW, j = j0 + q % TW;
-    if (i <= a.nr && j < a.ns) {
-      const long l = (long)i * a.ns + j;
-      s = a.dens[l];
-      vr = a.vr[l];
-      vt = a.vt[l];
-    }
+// staged position (row r, column cc of the TH+6 x HC stage) of halo cell h
+// of a tile of TH rows: the HALO_R rows above, the HALO_B rows below, then
+// the HALO_C columns on either side of each tile row
+template <int TH_>
+__device__ __forceinline__ void halo_cell(int h, int &r, int &cc) {
+  constexpr int NTOP = HALO_R * HC, NBOT = HALO_B * HC;
+  if (h < NTOP) {
+    r = h / HC;
+    cc = h % HC;
+  } else if (h < NTOP + NBOT) {
+    h -= NTOP;
+    r = HALO_R + TH_ + h / HC;
+    cc = h % HC;
+  } else {
+    h -= NTOP + NBOT;
+    const int c = h % (2 * HALO_C);
+    r = HALO_R + h / (2 * HALO_C);
+    cc = c < HALO_C ? c : c + TW;
   }
 }
 
@@ -1719,15 +1831,36 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   __shared__ double ct[2][HC];
 
   // ---- S0: load, CorrectVtheta, damping #1 (FP64), pressure, potential ----
-  // all of a thread's loads are issued before any of them is used
+  // all of a thread's loads are issued before any of them is used.  The
+  // tile's own cells are loaded in the mapping of the final stage (S5) and
+  // their adjusted FP64 state stays in registers until then; the halo
+  // cells (rows above and below, columns on either side) separately
+  constexpr int NH = hr * HC - th * TW, HPT = (NH + SRC32_TPB - 1) / SRC32_TPB;
+  double os[SRC32_OWN_MAX], ovr[SRC32_OWN_MAX], ovt[SRC32_OWN_MAX];
   {
-    double ls[SRC32_PER_THREAD], lvr[SRC32_PER_THREAD], lvt[SRC32_PER_THREAD];
+    double ls[HPT], lvr[HPT], lvt[HPT];
 #pragma unroll
-    for (int k = 0; k < SRC32_PER_THREAD; k++) {
+    for (int k = 0; k < SRC32_OWN_MAX; k++) {
       const int q = tid + k * SRC32_TPB;
+      os[k] = ovr[k] = ovt[k] = 0.0;
+      if (q < th * TW) {
+        const int r = q / TW + HALO_R, cc = q % TW + HALO_C, i = GI(r);
+        if (i >= 0 && i <= nr) {
+          const long l = (long)i * ns + GJ(cc);
+          os[k] = a.dens[l];
+          ovr[k] = a.vr[l];
+          ovt[k] = a.vt[l];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < HPT; k++) {
+      const int h = tid + k * SRC32_TPB;
       ls[k] = lvr[k] = lvt[k] = 0.0;
-      if (q < hr * HC) {
-        const int r = q / HC, cc = q - r * HC, i = GI(r);
+      if (h < NH) {
+        int r, cc;
+        halo_cell<th>(h, r, cc);
+        const int i = GI(r);
         if (i >= 0 && i <= nr) {
           const long l = (long)i * ns + GJ(cc);
           ls[k] = a.dens[l];
@@ -1769,31 +1902,42 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
       }
     }
     __syncthreads();
-#pragma unroll
-  for (int k = 0; k < SRC32_PER_THREAD; k++) {
-    const int q = tid + k * SRC32_TPB;
-    if (q >= hr * HC) break;
-    int r = q / HC, cc = q - r * HC;
-    int i = GI(r);
-    float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
-    if (i >= 0 && i <= nr) {
-      double sd = ls[k], vrd = lvr[k], vtd = lvt[k];
-      src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], evc[r], sd, vrd, vtd);
-      s = (float)sd;
-      vr = (float)vrd;
-      vt = (float)(vtd - rt[RT_VREF][r]);
-      if (i < nr) {
-        const double cs = rt[RT_CS][r];
-        p = (float)(sd * cs * cs);
-        phi = cell_potential_planets_f(a.P, rt[RT_MED][r], ct[0][cc], ct[1][cc], a.indirect);
+    // CorrectVtheta and damping #1 of a staged cell (the FP64 values in
+    // place), then its FP32 stage values
+    auto stage = [&](int r, int cc, double &sd, double &vrd, double &vtd) {
+      const int i = GI(r);
+      float s = 0.f, vr = 0.f, vt = 0.f, p = 0.f, phi = 0.f;
+      if (i >= 0 && i <= nr) {
+        src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], evc[r], sd, vrd, vtd);
+        s = (float)sd;
+        vr = (float)vrd;
+        vt = (float)(vtd - rt[RT_VREF][r]);
+        if (i < nr) {
+          const double cs = rt[RT_CS][r];
+          p = (float)(sd * cs * cs);
+          phi = cell_potential_planets_f(a.P, rt[RT_MED][r], ct[0][cc], ct[1][cc], a.indirect);
+        }
       }
+      sS[r][cc] = s;
+      sA[r][cc] = p;
+      sB[r][cc] = phi;
+      sC[r][cc] = vt;
+      sD[r][cc] = vr;
+    };
+#pragma unroll
+    for (int k = 0; k < SRC32_OWN_MAX; k++) {
+      const int q = tid + k * SRC32_TPB;
+      if (q >= th * TW) break;
+      stage(q / TW + HALO_R, q % TW + HALO_C, os[k], ovr[k], ovt[k]);
     }
-    sS[r][cc] = s;
-    sA[r][cc] = p;
-    sB[r][cc] = phi;
-    sC[r][cc] = vt;
-    sD[r][cc] = vr;
-  }
+#pragma unroll
+    for (int k = 0; k < HPT; k++) {
+      const int h = tid + k * SRC32_TPB;
+      if (h >= NH) break;
+      int r, cc;
+      halo_cell<th>(h, r, cc);
+      stage(r, cc, ls[k], lvr[k], lvt[k]);
+    }
   }
   __syncthreads();
 
@@ -1920,9 +2064,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   __syncthreads();
 
   // ---- S5: velocity update ("kernel", Kernels.cu:438-479), damping #2 ----
-  double os[SRC32_OWN_MAX], ovr[SRC32_OWN_MAX], ovt[SRC32_OWN_MAX];
-#pragma unroll
-  for (int k = 0; k < SRC32_OWN_MAX; k++) own_load(a, i0, j0, th, tid + k * SRC32_TPB, os[k], ovr[k], ovt[k]);
+  // (the own cells' state as adjusted by S0 is still in os, ovr, ovt)
 #pragma unroll
   for (int k = 0; k < SRC32_OWN_MAX; k++) {
     const int q = tid + k * SRC32_TPB;
@@ -1934,7 +2076,6 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
     long l = (long)i * ns + j;
     double s = os[k], vr = ovr[k], vt = ovt[k];
     const double *ev = evc[r];
-    src32_adjust(a, i, rt[RT_MED][r], rt[RT_EVZ][r], rt[RT_SMED][r], ev, s, vr, vt);
     if (a.opress && i < nr) a.opress[l] = s * rt[RT_CS][r] * rt[RT_CS][r];
     if (i == nr) {  // ghost ring (never read)
       a.odens[l] = 0.0f;
@@ -1954,9 +2095,13 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
       vtnew = damp_toward(vtnew, ev[0], ev[2]);
       s = damp_toward(s, ev[0], rt[RT_SMED][r]);
     }
-    a.odens[l] = (float)(s - rt[RT_SBAR][r]);
-    a.ovr[l] = (float)vrnew;
-    a.ovt[l] = (float)(vtnew - rt[RT_VREF][r]);
+    const double xs = s - rt[RT_SBAR][r], xv = vtnew - rt[RT_VREF][r];
+    const float hs = (float)xs, hv = (float)vrnew, ht = (float)xv;
+    a.odens[l] = hs;
+    a.ovr[l] = hv;
+    a.ovt[l] = ht;
+    if (in_lob(i, a.lob0, a.lob1))  // Sigma as a float pair near the planets
+      a.odens[(long)(nr + 1) * ns + l] = (float)(xs - (double)hs);
   }
 #undef GI
 #undef GJ
@@ -2000,8 +2145,13 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   bg_lanes bg;
   bg.load(a.sbar, a.vref, rt.r0, nr);
   // ring m's inputs are loaded one iteration ahead, unconditionally (as
-  // in k_trad), as the stored FP32 offsets
-  float n_s, n_v0, n_v1, n_vtp, n_vt;
+  // in k_trad), as the stored FP32 offsets.  Iterations that can touch a
+  // ring stored as float pairs (LO: m in [lob0 - 2, lob1 + 2)) also load
+  // the residuals, for every ring (the ghost ring's zero slots outside the
+  // band), and store them; the other iterations issue neither
+  const long S = (long)(nr + 1) * ns, zslot = S + (long)nr * ns;
+  auto lo_at = [&](int ring, long l) { return in_lob(ring, a.lob0, a.lob1) ? S + l : zslot + (l - (long)ring * ns); };
+  float n_s, n_v0, n_v1, n_vtp, n_vt, n_sl;
   {
     const long l = (long)m0 * ns + j;
     n_s = a.fdens[l];
@@ -2009,19 +2159,26 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     n_v1 = a.fvr[l + ns];
     n_vtp = a.fvt[(long)m0 * ns + jp];
     n_vt = a.fvt[l];
-    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
+    n_sl = a.fdens[lo_at(m0, l)];
+    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt), "v"(n_sl));
   }
-  auto ring = [&](int m, auto store) {
+  auto ring = [&](int m, auto store, auto LO) {
+    constexpr bool lo = decltype(LO)::value;
     const int r = rt.at(t, m, nr);
     if (bg.r0 != rt.r0) bg.load(a.sbar, a.vref, rt.r0, nr);
     const float c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
+    // the residual of ring m's Sigma: read only in LO iterations (rings
+    // outside LO iterations are outside the band)
+    const float c_sl = lo ? n_sl : 0.f;
     {
-      const long l = (long)min(m + 1, nr - 1) * ns;
+      const int mp = min(m + 1, nr - 1);
+      const long l = (long)mp * ns;
       n_s = a.fdens[l + j];
       n_v0 = c_v1;
       n_v1 = a.fvr[l + ns + j];
       n_vtp = a.fvt[l + jp];
       n_vt = a.fvt[l + j];
+      n_sl = lo ? a.fdens[lo_at(mp, l + j)] : 0.f;
     }
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
@@ -2036,7 +2193,7 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     const double Rm = lane_get(rt.Rmed, r);
     if (m < nr) {
       const double vb = lane_get(bg.vb, r);
-      const double s = lane_get(bg.sb, r) + (double)c_s;
+      const double s = rec32(lane_get(bg.sb, r), c_s, c_sl);
       const double v0 = (double)c_v0, v1 = (double)c_v1;
       Q[0][2] = s * v1;
       Q[1][2] = s * v0;
@@ -2085,25 +2242,37 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
     if (decltype(store)::value) {
       // ring o = k - 1 is complete: stored as FP32 offsets from its backgrounds
       const int o = k - 1;
-      const long lo = (long)o * ns + j;
+      const long lo_ = (long)o * ns + j;
       const float isurf = (float)lane_get(rt.isurf, r - 2);
       const double sb = lane_get(bg.sb, r - 2);
       double val[NQ];
       for (int q = 0; q < NQ; q++) val[q] = Q[q][0] + (double)((F[q][0] - F[q][1]) * isurf);
       // Sigma as the reader will reconstruct it, the angular momenta's background
-      const float fs = (float)(val[4] - sb);
-      const double lb = fp32_bg_angmom(sb + (double)fs, lane_get(bg.vb, r - 2), lane_get(rt.Rmed, r - 2), OF);
-      a.fout[0][lo] = (float)val[0];
-      a.fout[1][lo] = (float)val[1];
-      a.fout[2][lo] = (float)(val[2] - lb);
-      a.fout[3][lo] = (float)(val[3] - lb);
-      a.fout[4][lo] = fs;
+      const double xs = val[4] - sb;
+      const float fs = (float)xs, fsl = (float)(xs - (double)fs);
+      const float fsr = (lo && in_lob(o, a.lob0, a.lob1)) ? fsl : 0.f;
+      const double lb = fp32_bg_angmom(rec32(sb, fs, fsr), lane_get(bg.vb, r - 2), lane_get(rt.Rmed, r - 2), OF);
+      a.fout[0][lo_] = (float)val[0];
+      a.fout[1][lo_] = (float)val[1];
+      a.fout[2][lo_] = (float)(val[2] - lb);
+      a.fout[3][lo_] = (float)(val[3] - lb);
+      a.fout[4][lo_] = fs;
+      if (lo) a.fout[4][S + lo_] = fsl;  // read by k_taz only inside the band
     }
   };
+  // the LO iterations [L0, L1) of this strip (none when the band is empty)
+  const bool any_lo = a.lob0 < a.lob1;
+  const int L0 = any_lo ? a.lob0 - 2 : INT_MAX, L1 = any_lo ? a.lob1 + 2 : INT_MAX;
   int m = m0;
-  for (; m < b0 + 2; m++) ring(m, std::false_type());
+  auto run = [&](int to, auto store) {
+    const int x0 = min(max(L0, m), to), x1 = min(max(L1, x0), to);
+    for (; m < x0; m++) ring(m, store, std::false_type());
+    for (; m < x1; m++) ring(m, store, std::true_type());
+    for (; m < to; m++) ring(m, store, std::false_type());
+  };
+  run(b0 + 2, std::false_type());
   asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
-  for (; m <= b1 + 1; m++) ring(m, std::true_type());
+  run(b1 + 2, std::true_type());
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
@@ -2253,7 +2422,7 @@ static fg_status fused_supported(fg_ctx *c) {
 __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, double *cfl,
                                                          double *dt1d, const double *vmoy, const double *newdt,
                                                          const double *invRmed, double coef, double DeltaT,
-                                                         int nr) {
+                                                         int nr, double seq) {
   __shared__ double p[2 * FG_RED_BLOCKS];
   __shared__ double wmin[4];
   const int tid = threadIdx.x;
@@ -2263,15 +2432,7 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
     double s = seq_sum_lds(p + tid * B, B);
     out[tid] = 0.0 + s;
   } else if (tid >= 64) {
-    double best = 1e300;
-    for (int i = tid - 64; i < nr; i += 192) {
-      if (i >= 1 && newdt[i] < best) best = newdt[i];
-      if (i < nr - 1) {
-        double d = coef / fabs(vmoy[i] * invRmed[i] - vmoy[i + 1] * invRmed[i + 1]);
-        dt1d[i] = d;
-        if (d < best) best = d;
-      }
-    }
+    double best = cfl_ring_min(dt1d, vmoy, newdt, invRmed, coef, nr, tid - 64, 192);
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_down(best, off, 64);
       if (o < best) best = o;
@@ -2286,6 +2447,7 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
     if (DeltaT < newdt_) newdt_ = DeltaT;
     cfl[0] = (double)(int)(ceil(DeltaT / newdt_));
   }
+  seq_store(out, seq);
 }
 
 // results of the reductions for the next substep: CFL count and the disk
@@ -2293,10 +2455,13 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
 // ring_partials: the star force comes from k_fin's per-ring compensated
 // partials (FP32 mode) instead of the DeviceReduce-order sum
 static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials = false) {
+  const double seq = (double)++c->fu_seq;
+  c->fu_seq_last = 1;
   if (ring_partials) {
     prof_mark(c);  // (no separate force phase)
     hipLaunchKernelGGL(k_force_rings_cfl, dim3(1), dim3(256), 0, c->stream, c->fu_fpart, c->nr, c->d_fused,
-                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT);
+                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT,
+                       seq);
     FG_LAUNCH_CHECK();
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
@@ -2305,7 +2470,8 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
     if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
     prof_mark(c);
     hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->d_fused,
-                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr);
+                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr,
+                       seq);
     FG_LAUNCH_CHECK();
   }
   prof_mark(c);
@@ -2368,6 +2534,7 @@ fg_status fg_fused_planet_force_async(fg_ctx *c, int k, double x, double y, doub
     fg_set_error("fused planet force: no result block or planet %d", k);
     return FG_E_INVALID;
   }
+  c->fu_seq_last = 0;  // the results now wait for this reduction too
   return fg_disk_force_async(c, c->f[FG_DENS], x, y, rs, c->d_fused + FG_FUSED_PF_SLOT + 2 * k);
 }
 
@@ -2382,8 +2549,29 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
     fg_set_error("fg_fused_results: no fused reductions were launched (fg_fused_prepare or fg_step_fused first)");
     return FG_E_INVALID;
   }
-  FG_HIP(hipStreamSynchronize(c->stream));
-  prof_fold(c);
+  if (c->fu_seq_last && !c->prof_on) {
+    // only the final kernel is awaited: watch the slot it writes last
+    // (quicker than the stream's completion signal), asking the stream
+    // every 1024 polls so that a failed launch still ends the wait
+    volatile double *slot = c->h_fused + FG_FUSED_SEQ_SLOT;
+    const double want = (double)c->fu_seq;
+    for (unsigned n = 1; *slot != want; n++) {
+      if ((n & 1023) == 0) {
+        const hipError_t e = hipStreamQuery(c->stream);
+        if (e == hipErrorNotReady) continue;
+        FG_HIP(e);
+        if (*slot != want) {
+          fg_set_error("fg_fused_results: the final reduction finished without reporting");
+          return FG_E_HIP;
+        }
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  } else {
+    FG_HIP(hipStreamSynchronize(c->stream));
+    prof_fold(c);
+  }
   const double cfl = c->h_fused[FG_FUSED_CFL_SLOT];
   const double f[2] = {c->h_fused[0], c->h_fused[1]};
   if (fg_status s = fg_fused_check(c, cfl, f)) return s;
@@ -2394,6 +2582,30 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
   }
   if (lost) *lost = 0.0;  // evanescent edges: no outflow is recorded
   return FG_OK;
+}
+
+// FP32 bulk mode: the rings stored as float pairs, [*b0, *b1): those whose
+// centre lies within FP32_PAIR_HILL Hill radii (radially) of a planet, where
+// the circumplanetary material stays and the storage rounding would add up
+// (empty without planets)
+#define FP32_PAIR_HILL 1.0
+static void fp32_pair_rings(const fg_ctx *c, const fg_planet_pot *pl, int nb, int *b0, int *b1) {
+  int lo = c->nr, hi = 0;
+  for (int k = 0; k < nb; k++) {
+    const double rp = sqrt(pl[k].x * pl[k].x + pl[k].y * pl[k].y);
+    if (!(pl[k].mass > 0.0) || !(rp > 0.0)) continue;
+    const double w = FP32_PAIR_HILL * rp * cbrt(pl[k].mass / 3.0);
+    for (int i = 0; i < c->nr; i++) {
+      const double r = c->h_Rmed[i];
+      if (r >= rp - w && r <= rp + w) {
+        lo = std::min(lo, i);
+        hi = std::max(hi, i + 1);
+      }
+    }
+  }
+  if (lo >= hi) lo = hi = 0;
+  *b0 = lo;
+  *b1 = hi;
 }
 
 extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, double domega,
@@ -2459,9 +2671,13 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     c->prof_pending = 1;
   }
   const bool fp32 = c->p.precision == FG_PREC_FP32;
+  int lob0 = 0, lob1 = 0;
+  if (fp32) fp32_pair_rings(c, planets, nb, &lob0, &lob1);
   prof_mark(c);
   if (fp32) {
     src32_args s3;
+    s3.lob0 = lob0;
+    s3.lob1 = lob1;
     s3.dens = sa.dens;
     s3.vr = sa.vr;
     s3.vt = sa.vt;
@@ -2550,6 +2766,8 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   ta.vmed = c->VMed;
   ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
   ta.band = c->tile_trad_band;
+  ta.lob0 = lob0;
+  ta.lob1 = lob1;
   prof_mark(c);
   const dim3 gtrad(nr + ta.nstrips * ((nr + ta.band - 1) / ta.band));
   if (fp32)
@@ -2589,6 +2807,8 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.nr = nr;
   za.ns = ns;
   za.fast = c->p.fast_transport;
+  za.lob0 = lob0;
+  za.lob1 = lob1;
   prof_mark(c);
   if (adi) {
     // six quantities: CL = 4 keeps two waves per SIMD
@@ -2646,6 +2866,8 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   fa.fxrmm = b.fXrmm();
   fa.sbar = c->fu_sbar[c->fu_sbar_cur];
   fa.sbar_next = c->fu_sbar[c->fu_sbar_cur ^ 1];
+  fa.lob0 = lob0;
+  fa.lob1 = lob1;
   prof_mark(c);
   const dim3 gfin(FG_NXCD * ((nr + FG_NXCD - 1) / FG_NXCD));
   if (adi)
@@ -2740,11 +2962,16 @@ __global__ __launch_bounds__(TRAD_TPB) void k_batch_trad(const batch_disk *__res
 
 // one chunk of one ring per item: y = ring * chunks + chunk
 __global__ __launch_bounds__(64) void k_batch_taz(const batch_disk *__restrict__ D, const int2 *__restrict__ items, int n) {
+  __shared__ tazw_lds<4, 5> sh;
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
     if (D[w.x].active) {
       const int nch = (D[w.x].taz.ns + TAZW_P32 - 1) / TAZW_P32;
-      tazw_body<4, true>(D[w.x].taz, w.y / nch, w.y % nch);
+      const int i = w.y / nch;
+      if (in_lob(i, D[w.x].taz.lob0, D[w.x].taz.lob1))
+        tazw_body<4, true, 5, true>(D[w.x].taz, i, w.y % nch, sh);
+      else
+        tazw_body<4, true, 5, false>(D[w.x].taz, i, w.y % nch, sh);
     }
     __syncthreads();
   }
@@ -2754,7 +2981,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   extern __shared__ double lds_bfin[];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = uniform2(items[it]);
-    if (D[w.x].active) fin_ring<1, false, true>(D[w.x].fin, w.y, lds_bfin);
+    if (D[w.x].active) {
+      const fin_args &fa = D[w.x].fin;
+      if (in_lob(w.y, fa.lob0 - 1, fa.lob1 + 1))
+        fin_ring<1, false, true, true>(fa, w.y, lds_bfin);
+      else
+        fin_ring<1, false, true, false>(fa, w.y, lds_bfin);
+    }
     __syncthreads();
   }
 }
@@ -2959,7 +3192,11 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fg_ctx *c = b->ctx[k];
   const int nr = c->nr, ns = c->ns;
   fused_bufs fb = bufs_of(c);
+  int lob0 = 0, lob1 = 0;
+  fp32_pair_rings(c, planets, nb, &lob0, &lob1);
   src32_args &s = d.src;
+  s.lob0 = lob0;
+  s.lob1 = lob1;
   s.dens = c->f[FG_DENS];
   s.vr = c->f[FG_VRAD];
   s.vt = c->f[FG_VTHETA];
@@ -3017,6 +3254,8 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   ta.ns = ns;
   ta.nstrips = (ns + TRAD_TPB - 1) / TRAD_TPB;
   ta.band = c->tile_trad_band;
+  ta.lob0 = lob0;
+  ta.lob1 = lob1;
   taz_args &za = d.taz;
   za.vt = fb.Bvt;
   za.vmed = c->VMed;
@@ -3044,6 +3283,8 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   za.nr = nr;
   za.ns = ns;
   za.fast = c->p.fast_transport;
+  za.lob0 = lob0;
+  za.lob1 = lob1;
   fin_args &fa = d.fin;
   fa.xd = fb.Xd;
   fa.xvt = fb.Xvt;
@@ -3077,6 +3318,8 @@ fg_status fg_batch_dev_set(fg_batch_dev *b, int k, int active, double dt, double
   fa.fxrmm = fb.fXrmm();
   fa.sbar = sbar;
   fa.sbar_next = sbar_next;
+  fa.lob0 = lob0;
+  fa.lob1 = lob1;
   d.force = force_src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, ns};
   d.ncell = (long)nr * ns;
   d.B = red_blocks(d.ncell);

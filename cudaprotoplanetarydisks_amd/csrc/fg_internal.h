@@ -33,8 +33,11 @@ fg_status fg_viscosity_into(fg_ctx *c, double *nu_out);
 // that k_fin and the CFL ring pass set to 1 when they meet a negative or
 // non-finite density, and from FG_FUSED_PF_SLOT the disk force on each
 // planet that feels the disk (two doubles per planet), launched behind a
-// substep for the next one.
+// substep for the next one.  FG_FUSED_SEQ_SLOT: the final kernel writes the
+// sequence number of its launch there last, after a system-scope fence,
+// so the host can wait for the results by watching that slot.
 #define FG_FUSED_PF_SLOT 8
+#define FG_FUSED_SEQ_SLOT 5
 #define FG_FUSED_SLOTS (FG_FUSED_PF_SLOT + 2 * FG_MAX_PLANETS)
 #define FG_FUSED_CFL_SLOT 6
 #define FG_FUSED_CRASH_SLOT 7
@@ -167,6 +170,8 @@ struct fg_ctx {
   // fused path's force and CFL count: written by the final reduction kernel
   // straight into coherent pinned host memory (no copy); device alias
   double *h_fused, *d_fused;
+  unsigned long long fu_seq;  // sequence number of the last final-kernel launch
+  int fu_seq_last;            // nothing the results depend on is queued behind it
   // Hill-cut force variants (fg_disk_force_hill), allocated on first use
   double *hill_partial;  // 4 * FG_HILL_MAX * FG_RED_BLOCKS block sums
   double *hill_out;      // 4 * FG_HILL_MAX results

@@ -276,32 +276,78 @@ __device__ __forceinline__ void dreduce_final(const double *partial, int B, doub
 }
 
 
-// ConditionCFLKernel2D3 (Kernels.cu:1337-1360) and the host side of
-// ConditionCFL (SourceEuler.cu:520-546): min of the ring minima and the
-// 1D shear limits, then the sub-cycle count ceil(DeltaT / dt)
-// cfl: an int, or a double slot of a result block copied to the host at once
-template <class T>
-__device__ __forceinline__ void cfl_final(T *cfl, double *dt1d, const double *vmoy, const double *newdt,
-                                          const double *invRmed, double coef, double DeltaT, int nr) {
-  __shared__ double red[256];
+// One thread's part of ConditionCFLKernel2D3 (Kernels.cu:1337-1360): the
+// minimum of the ring minima newdt[i] (i >= 1) and the 1D shear limits
+// dt1d[i] (stored) over rings i = i0, i0 + stride, ...  The loads of four
+// rings are issued before their dt1d stores (the compiler cannot move a load
+// across a store to a pointer that might alias it), so a thread waits for
+// memory once per four rings; a minimum does not depend on the order of its
+// comparisons.
+__device__ __forceinline__ double cfl_ring_min(double *dt1d, const double *vmoy, const double *newdt,
+                                               const double *invRmed, double coef, int nr, int i0, int stride) {
   double best = 1e300;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    if (i >= 1 && newdt[i] < best) best = newdt[i];
-    if (i < nr - 1) {
-      double d = coef / fabs(vmoy[i] * invRmed[i] - vmoy[i + 1] * invRmed[i + 1]);
-      dt1d[i] = d;
-      if (d < best) best = d;
+  for (int ib = i0; ib < nr; ib += 4 * stride) {
+    double nd[4], v0[4], v1[4], r0[4], r1[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = ib + u * stride;
+      nd[u] = v0[u] = v1[u] = r0[u] = r1[u] = 0.0;
+      if (i < nr) {
+        nd[u] = newdt[i];
+        v0[u] = vmoy[i];
+        r0[u] = invRmed[i];
+        if (i < nr - 1) {
+          v1[u] = vmoy[i + 1];
+          r1[u] = invRmed[i + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = ib + u * stride;
+      if (i >= nr) break;
+      if (i >= 1 && nd[u] < best) best = nd[u];
+      if (i < nr - 1) {
+        const double d = coef / fabs(v0[u] * r0[u] - v1[u] * r1[u]);
+        dt1d[i] = d;
+        if (d < best) best = d;
+      }
     }
   }
-  red[threadIdx.x] = best;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s && red[threadIdx.x + s] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
-    __syncthreads();
+  return best;
+}
+
+// minimum over a 256-thread block into every thread's return value
+// (shuffles within each wave, then the four wave minima)
+__device__ __forceinline__ double block_min256(double best) {
+  __shared__ double wmin[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(best, off, 64);
+    if (o < best) best = o;
   }
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+  __syncthreads();
+  double m = wmin[0];
+  for (int w = 1; w < 4; w++)
+    if (wmin[w] < m) m = wmin[w];
+  return m;
+}
+
+// ConditionCFLKernel2D3 and the host side of ConditionCFL
+// (SourceEuler.cu:520-546) in one 256-thread block: min of the ring minima
+// and the 1D shear limits, then the sub-cycle count ceil(DeltaT / dt)
+// cfl: an int, or a double slot of a result block copied to the host at once
+template <class T>
+__device__ __forceinline__ void cfl_count_store(T *cfl, double best, double DeltaT) {
   if (threadIdx.x == 0) {
-    double newdt_ = red[0];
+    double newdt_ = best;
     if (DeltaT < newdt_) newdt_ = DeltaT;
     cfl[0] = (T)(int)(ceil(DeltaT / newdt_));
   }
+}
+template <class T>
+__device__ __forceinline__ void cfl_final(T *cfl, double *dt1d, const double *vmoy, const double *newdt,
+                                          const double *invRmed, double coef, double DeltaT, int nr) {
+  const double best = block_min256(cfl_ring_min(dt1d, vmoy, newdt, invRmed, coef, nr, threadIdx.x, 256));
+  cfl_count_store(cfl, best, DeltaT);
 }
