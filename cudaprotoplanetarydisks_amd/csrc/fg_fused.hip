@@ -49,7 +49,6 @@
 #define SRC64_TPB 512  // k_src: two 80 KB workgroups per CU, four waves per SIMD
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
-fg_status fg_disk_force_partial_async(fg_ctx *c, const double *dens, double x, double y, double rs, int *nblocks);
 
 
 
@@ -2416,14 +2415,14 @@ static fg_status fused_supported(fg_ctx *c) {
 // The host loop of DeviceReduce for the star force (dreduce_final: lanes 0
 // and 1 add the B block sums of x and y in order, then +0.0 for the inner
 // sum, SideEuler.cu:144) and, in waves 1-3 meanwhile, the CFL count of the
-// same substep (cfl_final: DT1D of every ring, Kernels.cu:1331-1337, and
-// the minimum with the ring minima, Kernels.cu:1337-1359; a minimum does
-// not depend on the order its candidates are compared in).
-__global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, double *cfl,
-                                                         double *dt1d, const double *vmoy, const double *newdt,
-                                                         const double *invRmed, double coef, double DeltaT,
-                                                         int nr, double seq) {
-  __shared__ double p[2 * FG_RED_BLOCKS];
+// same substep (DT1D of every ring, Kernels.cu:1331-1337, and the minimum
+// with the ring minima, Kernels.cu:1337-1359; a minimum does not depend on
+// the order its candidates are compared in), then the result block's
+// sequence slot.  p: LDS for the 2B block sums; every thread of a
+// 256-thread block calls it.
+__device__ __forceinline__ void force_cfl_final(const double *partial, int B, double *out, double *cfl, double *dt1d,
+                                                const double *vmoy, const double *newdt, const double *invRmed,
+                                                double coef, double DeltaT, int nr, double seq, double *p) {
   __shared__ double wmin[4];
   const int tid = threadIdx.x;
   for (int k = tid; k < 2 * B; k += 256) p[k] = partial[k];
@@ -2450,6 +2449,20 @@ __global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, 
   seq_store(out, seq);
 }
 
+// the DeviceReduce-order partials of the star force (dreduce_block)
+template <int CH>
+__global__ __launch_bounds__(RED_TPB) void k_dreduce_partial_f(force_src src, long n, int B, double *partial) {
+  dreduce_block<force_src, CH>(src, n, B, blockIdx.x, partial);
+}
+
+__global__ __launch_bounds__(256) void k_force_cfl_final(const double *partial, int B, double *out, double *cfl,
+                                                         double *dt1d, const double *vmoy, const double *newdt,
+                                                         const double *invRmed, double coef, double DeltaT,
+                                                         int nr, double seq) {
+  __shared__ double p[2 * FG_RED_BLOCKS];
+  force_cfl_final(partial, B, out, cfl, dt1d, vmoy, newdt, invRmed, coef, DeltaT, nr, seq, p);
+}
+
 // results of the reductions for the next substep: CFL count and the disk
 // force on the star; copied to pinned memory behind an event
 // ring_partials: the star force comes from k_fin's per-ring compensated
@@ -2466,12 +2479,16 @@ static fg_status fused_reductions(fg_ctx *c, double DeltaT, bool ring_partials =
   } else {
     // DeviceReduce partials, then one kernel for their final sum and the
     // CFL count
-    int B = 0;
-    if (fg_status s = fg_disk_force_partial_async(c, c->f[FG_DENS], 0.0, 0.0, 0.0, &B)) return s;
+    const long n = (long)c->nr * c->ns;
+    const int B = red_blocks(n);
+    force_src src{c->f[FG_DENS], c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, 0.0, 0.0, 0.0, c->ns};
+    hipLaunchKernelGGL((k_dreduce_partial_f<RED_CH>), dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B,
+                       c->red_partial);
+    FG_LAUNCH_CHECK();
     prof_mark(c);
     hipLaunchKernelGGL(k_force_cfl_final, dim3(1), dim3(256), 0, c->stream, c->red_partial, B, c->d_fused,
-                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef, DeltaT, c->nr,
-                       seq);
+                       c->d_fused + FG_FUSED_CFL_SLOT, c->DT1D, c->Vmoy, c->newDT, c->t.invRmed, c->t.dt1d_coef,
+                       DeltaT, c->nr, seq);
     FG_LAUNCH_CHECK();
   }
   prof_mark(c);

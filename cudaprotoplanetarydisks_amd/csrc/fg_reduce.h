@@ -49,10 +49,24 @@ struct force_src {
   const double *dens, *Surf, *Rmed, *cosc, *sinc;
   double x, y, rs;
   int ns;
+  static constexpr bool kRow = true;
   template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
     int i, j;
     cell_ij<NARROW>(e, ns, i, j);
+    summand(e, i, j, fx, fy);
+  }
+  // cell e0 + lane of a 64-cell row that lies in one ring (NSEC a multiple
+  // of 64, e0 a multiple of 64 and the same across the wave): the ring's
+  // index and values are the wave's, no per-lane division
+  __device__ __forceinline__ void row_cell(long e0, int lane, double &fx, double &fy) const {
+    int i, j0;
+    cell_ij(e0, ns, i, j0);
+    i = __builtin_amdgcn_readfirstlane(i);
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    summand(e0 + lane, i, j0 + lane, fx, fy);
+  }
+  __device__ __forceinline__ void summand(long e, int i, int j, double &fx, double &fy) const {
     double cellmass = Surf[i] * dens[e];
     double dx = Rmed[i] * cosc[j] - x;
     double dy = Rmed[i] * sinc[j] - y;
@@ -104,6 +118,7 @@ struct cpm_src {
   const double *dens, *Surf, *Rmed, *cosc, *sinc;
   double xpl, ypl, hill;
   int ns;
+  static constexpr bool kRow = false;
   template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &m, double &zero) const {
     int i, j;
@@ -118,6 +133,7 @@ struct cpm_src {
 
 struct array_src {
   const double *a, *b;  // b may be NULL
+  static constexpr bool kRow = false;
   template <bool NARROW = false>
   __device__ __forceinline__ void operator()(long e, double &fx, double &fy) const {
     fx = a[e];
@@ -131,19 +147,31 @@ struct array_src {
 // round's summands are evaluated branch-free (indices past n evaluate
 // cell n - 1 and are dropped), so all their loads issue together; wave 0
 // then reads its LDS values 8 rows ahead of the dependent adds.
-template <bool NARROW, class Src, int CH>
+// ROW: every 64-cell row lies in one ring (Src::row_cell); a row is then
+// wholly inside the grid or wholly past it, and a wave stages one row.
+template <bool NARROW, bool ROW, class Src, int CH>
 __device__ __forceinline__ void dreduce_stage(const Src &src, long n, long b64, long G, long k0, double (*vx)[64],
                                               double (*vy)[64]) {
 #pragma unroll
   for (int c0 = 0; c0 < CH * 64 / RED_TPB; c0++) {
     const int c = threadIdx.x + c0 * RED_TPB;
-    const int kk = c >> 6, lane = c & 63;
-    const long e = b64 + (k0 + kk) * G + lane;
-    const bool ok = e < n;
+    const int lane = c & 63;
     double fx, fy;
-    src.template operator()<NARROW>(ok ? e : n - 1, fx, fy);
-    vx[kk][lane] = ok ? fx : 0.0;
-    vy[kk][lane] = ok ? fy : 0.0;
+    if constexpr (ROW) {
+      const int kk = __builtin_amdgcn_readfirstlane(c >> 6);
+      const long e0 = b64 + (k0 + kk) * G;
+      const bool ok = e0 < n;
+      src.row_cell(ok ? e0 : n - 64, lane, fx, fy);
+      vx[kk][lane] = ok ? fx : 0.0;
+      vy[kk][lane] = ok ? fy : 0.0;
+    } else {
+      const int kk = c >> 6;
+      const long e = b64 + (k0 + kk) * G + lane;
+      const bool ok = e < n;
+      src.template operator()<NARROW>(ok ? e : n - 1, fx, fy);
+      vx[kk][lane] = ok ? fx : 0.0;
+      vy[kk][lane] = ok ? fy : 0.0;
+    }
   }
 }
 
@@ -155,12 +183,21 @@ __device__ __forceinline__ void dreduce_block(const Src &src, long n, int B, int
   const long G = 64L * B;
   const long b64 = 64L * blk;
   const bool narrow = n <= 0x80000000L;  // every cell index below 2^31
+  bool row = false;
+  if constexpr (Src::kRow) row = src.ns % 64 == 0;  // rows of 64 cells never cross a ring
   double acc = 0.0;
   for (long k0 = 0; b64 + k0 * G < n; k0 += CH) {
+    if constexpr (Src::kRow) {
+      if (row) {
+        dreduce_stage<true, true, Src, CH>(src, n, b64, G, k0, vx, vy);
+        goto staged;
+      }
+    }
     if (narrow)
-      dreduce_stage<true, Src, CH>(src, n, b64, G, k0, vx, vy);
+      dreduce_stage<true, false, Src, CH>(src, n, b64, G, k0, vx, vy);
     else
-      dreduce_stage<false, Src, CH>(src, n, b64, G, k0, vx, vy);
+      dreduce_stage<false, false, Src, CH>(src, n, b64, G, k0, vx, vy);
+  staged:
     __syncthreads();
     if (threadIdx.x < 64) {
       const int t = threadIdx.x & 31;

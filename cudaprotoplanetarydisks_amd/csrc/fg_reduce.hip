@@ -38,19 +38,6 @@ fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y,
   return dreduce(c, src, (long)c->nr * c->ns, out_dev, 1, 1);
 }
 
-// the DeviceReduce partials of the disk force only (their block sums are
-// completed by the caller's final kernel); *nblocks receives B
-fg_status fg_disk_force_partial_async(fg_ctx *c, const double *dens, double x, double y, double rs, int *nblocks) {
-  force_src src{dens, c->t.Surf, c->t.Rmed, c->t.cos_cell, c->t.sin_cell, x, y, rs, c->ns};
-  const long n = (long)c->nr * c->ns;
-  const int B = red_blocks(n);
-  hipLaunchKernelGGL((k_dreduce_partial<force_src, RED_CH>), dim3(B), dim3(RED_TPB), 0, c->stream, src, n, B,
-                     c->red_partial);
-  FG_LAUNCH_CHECK();
-  *nblocks = B;
-  return FG_OK;
-}
-
 extern "C" fg_status fg_disk_force(fg_ctx *c, double x, double y, double rsmoothing, double *out2) {
   if (!c || !out2) {
     fg_set_error("fg_disk_force: NULL argument");
