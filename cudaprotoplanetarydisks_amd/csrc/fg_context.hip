@@ -80,6 +80,7 @@ extern "C" fg_status fg_create(const fg_params *p, const fg_tables *t, int devic
   fg_ctx *c = new fg_ctx();
   c->tile_src_th = 16;
   c->tile_trad_band = 16;
+  c->fp32_pair_hill = 1.0;
   c->p = *p;
   c->nr = p->nrad;
   c->ns = p->nsec;

@@ -2602,16 +2602,15 @@ extern "C" fg_status fg_fused_results(fg_ctx *c, int *ncfl, double *force2, doub
 }
 
 // FP32 bulk mode: the rings stored as float pairs, [*b0, *b1): those whose
-// centre lies within FP32_PAIR_HILL Hill radii (radially) of a planet, where
-// the circumplanetary material stays and the storage rounding would add up
-// (empty without planets)
-#define FP32_PAIR_HILL 1.0
+// centre lies within fp32_pair_hill Hill radii (radially, 1 by default;
+// fg_set_fp32_pairs) of a planet, where the circumplanetary material stays
+// and the storage rounding would add up (empty without planets or at 0)
 static void fp32_pair_rings(const fg_ctx *c, const fg_planet_pot *pl, int nb, int *b0, int *b1) {
   int lo = c->nr, hi = 0;
-  for (int k = 0; k < nb; k++) {
+  for (int k = 0; c->fp32_pair_hill > 0.0 && k < nb; k++) {
     const double rp = sqrt(pl[k].x * pl[k].x + pl[k].y * pl[k].y);
     if (!(pl[k].mass > 0.0) || !(rp > 0.0)) continue;
-    const double w = FP32_PAIR_HILL * rp * cbrt(pl[k].mass / 3.0);
+    const double w = c->fp32_pair_hill * rp * cbrt(pl[k].mass / 3.0);
     for (int i = 0; i < c->nr; i++) {
       const double r = c->h_Rmed[i];
       if (r >= rp - w && r <= rp + w) {
@@ -3385,6 +3384,16 @@ void fg_batch_dev_invalidate(fg_batch_dev *b) {
     *q = nullptr;
   }
   b->ready = 0;
+}
+
+extern "C" fg_status fg_set_fp32_pairs(fg_ctx *c, double hill_radii) {
+  if (!c) return FG_E_INVALID;
+  if (!(hill_radii >= 0.0 && hill_radii <= 100.0)) {
+    fg_set_error("fg_set_fp32_pairs: %g Hill radii (0..100)", hill_radii);
+    return FG_E_INVALID;
+  }
+  c->fp32_pair_hill = hill_radii;
+  return FG_OK;
 }
 
 extern "C" fg_status fg_set_tiles(fg_ctx *c, int src_rows, int trad_band) {

@@ -158,6 +158,9 @@ struct fg_ctx {
   // tile shapes of the FP32 fused kernels (rings per k_src tile, rings
   // per radial-sweep band), set by fg_set_tiles
   int tile_src_th, tile_trad_band;
+  // FP32 bulk mode: Sigma stored as float pairs on the rings within this
+  // many Hill radii of a planet (fg_set_fp32_pairs)
+  double fp32_pair_hill;
   // fused-path scratch
   double *fu_mom[4];  // transported radial momenta per ring band
   double *fu_ring;    // per-ring partial results

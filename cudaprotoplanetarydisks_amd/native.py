@@ -86,6 +86,7 @@ SIGNATURES = {
     "fg_fused_prepare": (ctypes.c_int, [_vp, _d]),
     "fg_fused_results": (ctypes.c_int, [_vp, _pi, _pd, _pd]),
     "fg_set_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "fg_set_fp32_pairs": (ctypes.c_int, [_vp, ctypes.c_double]),
     "fg_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fg_profile_read": (ctypes.c_int, [_vp, _pd, ctypes.POINTER(ctypes.c_long), ctypes.c_int]),
     "fg_sim_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FgSimOptions), ctypes.POINTER(_vp)]),

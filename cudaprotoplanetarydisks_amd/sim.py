@@ -22,7 +22,7 @@ def _ptr(a):
 
 class Simulation:
     def __init__(self, parfile, device=0, precision="fp64", fused=False, write_files=False, outdir=None,
-                 cli_flags="", noise_seed=0, noise_amp=0.0, diagnostics=False):
+                 cli_flags="", noise_seed=0, noise_amp=0.0, diagnostics=False, fp32_pairs=None):
         opt = native.FgSimOptions()
         opt.device = device
         opt.precision = native.FG_PREC_FP32 if precision == "fp32" else native.FG_PREC_FP64
@@ -45,6 +45,8 @@ class Simulation:
             t = load_tiles().get(f"{self.nrad}x{self.nsec}")
             if t:
                 call("fg_set_tiles", lib().fg_sim_ctx(self.h), int(t["src_rows"]), int(t["trad_band"]))
+            if fp32_pairs is not None:  # Hill radii of float-pair Sigma storage (default 1)
+                call("fg_set_fp32_pairs", lib().fg_sim_ctx(self.h), float(fp32_pairs))
 
     def close(self):
         if getattr(self, "h", None):

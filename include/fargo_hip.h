@@ -243,6 +243,13 @@ fg_status fg_fused_results(fg_ctx *ctx, int *ncfl, double *force2, double *lost_
  * No reference counterpart (the reference's launch shape is fixed,
  * Main.cu:204-226). */
 fg_status fg_set_tiles(fg_ctx *ctx, int src_rows, int trad_band);
+/* FP32 bulk mode: Sigma, between the fused kernels, is stored as a pair of
+ * floats (high part and rounding residual) on the rings within hill_radii
+ * Hill radii (radially) of a planet, and as one float elsewhere; 1 by
+ * default, 0 stores single floats everywhere.  The pairs keep the rounding
+ * of the stored Sigma from adding up in circumplanetary material.  No
+ * reference counterpart (the reference computes in FP64 only). */
+fg_status fg_set_fp32_pairs(fg_ctx *ctx, double hill_radii);
 /* No counterpart in the reference: a self-check of the FP64 division with a
  * shared denominator that the fused kernels use (one reciprocal refinement
  * of d for several numerators).  Computes num[k] / den[k] both ways on the

@@ -94,3 +94,63 @@ def test_fp32_orbit_count_matches_time(deck):
             s.algogas()
         assert s.get("PhysicalTime") == pytest.approx(20 * s.get("DT"), rel=1e-12)
         assert s.get("PhysicalTime") / (2 * math.pi) == pytest.approx(1.0, rel=1e-9)
+
+
+def run_pair(d, pairs, n_dt):
+    with Simulation(d, precision="fp32", noise_seed=11, noise_amp=1e-3, fp32_pairs=pairs) as s:
+        for _ in range(n_dt):
+            s.algogas()
+        return s.state()
+
+
+def test_fp32_pairs_active_near_planet(deck, tmp_path):
+    """Sigma stored as float pairs near the planet changes the run (the pair
+    path is taken); with a massless planet the band is empty and the
+    setting changes nothing."""
+    from cudaprotoplanetarydisks_amd.workloads import write_planets
+    d = deck(Nrad=64, Nsec=192)
+    a, b = run_pair(d, 1.0, 6), run_pair(d, 0.0, 6)
+    assert not np.array_equal(a[0], b[0])
+    assert np.abs(a[0] - b[0]).max() <= 1e-5 * np.abs(b[0]).max()
+    cfg = write_planets(tmp_path / "massless.cfg", [("Z", 1.0, 0.0, 0.0, "NO", "NO")])
+    nop = deck(name="nop.par", Nrad=64, Nsec=192, PlanetConfig=cfg)
+    a, b = run_pair(nop, 1.0, 4), run_pair(nop, 0.0, 4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("where", ["inner", "outer", "whole"])
+def test_fp32_pair_band_at_grid_edges(deck, tmp_path, where):
+    """Float-pair bands that reach ring 0, the last ring, or every ring (the
+    q = 1e-3 planet at r = 1 with grids that end within the band) against
+    the FP64 run, the planet feeling the disk (the results then wait behind
+    its force reductions)."""
+    from cudaprotoplanetarydisks_amd.workloads import write_planets
+    rmin, rmax, hill = {"inner": (0.85, 2.5, 2.5), "outer": (0.4, 1.15, 2.5), "whole": (0.8, 1.2, 3.0)}[where]
+    cfg = write_planets(tmp_path / f"{where}.cfg", [("A", 1.0, 1e-3, 0.0, "YES", "NO")])
+    d = deck(name=f"{where}.par", Nrad=48, Nsec=160, Rmin=rmin, Rmax=rmax, PlanetConfig=cfg)
+    s64 = Simulation(d, fused=True, precision="fp64")
+    s32 = Simulation(d, precision="fp32", fp32_pairs=hill)
+    try:
+        for _ in range(6):
+            s64.algogas()
+            s32.algogas()
+        for x, y in zip(s32.state(), s64.state()):
+            x, y = x[:-1], y[:-1]
+            assert np.isfinite(x).all()
+            assert np.abs(x - y).max() <= 5e-5 * np.abs(y).max()
+    finally:
+        s64.close()
+        s32.close()
+
+
+def test_fp32_pairs_rejects_bad_width(deck):
+    from cudaprotoplanetarydisks_amd.native import lib
+    with Simulation(deck(), precision="fp32") as s:
+        with pytest.raises(FargoError):
+            call_pairs(lib().fg_sim_ctx(s.h), -1.0)
+
+
+def call_pairs(ctx, w):
+    from cudaprotoplanetarydisks_amd.native import call
+    call("fg_set_fp32_pairs", ctx, w)
