@@ -649,26 +649,23 @@ struct bg_lanes {
   }
 };
 
-// ring_mean_wave of a ring of vtheta stored as FP32 offsets from vb: the
-// same sequential sum of the FP64 values vb + offset
+// FP32 mode's VMed (ComputeAverageThetaVelocitiesKernel, Kernels.cu:
+// 1070-1081) of one ring by one wave: vtheta (background + stored offset)
+// summed per lane over sectors lane, lane + 64, ... (eight loads in flight),
+// then a shuffle tree.  The FP32 mode does not replay the reference's
+// one-thread order; the sum is no less accurate.
 __device__ void ring_mean_wave32(const float *v, double vb, int ns, double *out) {
-  __shared__ double buf[64 * RMW_CH];
   const int lane = threadIdx.x;
-  const int nch = (ns + 64 * RMW_CH - 1) / (64 * RMW_CH);
-  float nx[RMW_CH];
-#pragma unroll
-  for (int u = 0; u < RMW_CH; u++) nx[u] = (lane + 64 * u < ns) ? v[lane + 64 * u] : 0.0f;
   double sum = 0.0;
-  for (int c = 0; c < nch; c++) {
+  for (int j0 = lane; j0 < ns; j0 += 64 * 8) {
+    float x[8];
 #pragma unroll
-    for (int u = 0; u < RMW_CH; u++) buf[lane + 64 * u] = vb + (double)nx[u];
-    const int e0 = (c + 1) * 64 * RMW_CH + lane;
+    for (int u = 0; u < 8; u++) x[u] = (j0 + 64 * u < ns) ? v[j0 + 64 * u] : 0.0f;
 #pragma unroll
-    for (int u = 0; u < RMW_CH; u++) nx[u] = (e0 + 64 * u < ns) ? v[e0 + 64 * u] : 0.0f;
-    __syncthreads();
-    if (lane == 0) sum = seq_sum_lds(buf, min(64 * RMW_CH, ns - c * 64 * RMW_CH), sum);
-    __syncthreads();
+    for (int u = 0; u < 8; u++)
+      if (j0 + 64 * u < ns) sum += vb + (double)x[u];
   }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
   if (lane == 0) *out = sum / (double)ns;
 }
 
@@ -2143,42 +2140,45 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
   rt.load(t, m0 - 2, nr);
   bg_lanes bg;
   bg.load(a.sbar, a.vref, rt.r0, nr);
-  // ring m's inputs are loaded one iteration ahead, unconditionally (as
-  // in k_trad), as the stored FP32 offsets.  Iterations that can touch a
-  // ring stored as float pairs (LO: m in [lob0 - 2, lob1 + 2)) also load
-  // the residuals, for every ring (the ghost ring's zero slots outside the
-  // band), and store them; the other iterations issue neither
-  const long S = (long)(nr + 1) * ns, zslot = S + (long)nr * ns;
-  auto lo_at = [&](int ring, long l) { return in_lob(ring, a.lob0, a.lob1) ? S + l : zslot + (l - (long)ring * ns); };
-  float n_s, n_v0, n_v1, n_vtp, n_vt, n_sl;
-  {
-    const long l = (long)m0 * ns + j;
-    n_s = a.fdens[l];
-    n_v0 = a.fvr[l];
-    n_v1 = a.fvr[l + ns];
-    n_vtp = a.fvt[(long)m0 * ns + jp];
-    n_vt = a.fvt[l];
-    n_sl = a.fdens[lo_at(m0, l)];
-    asm volatile("" ::"v"(n_s), "v"(n_v0), "v"(n_v1), "v"(n_vtp), "v"(n_vt), "v"(n_sl));
-  }
-  auto ring = [&](int m, auto store, auto LO) {
-    constexpr bool lo = decltype(LO)::value;
+  // Every iteration issues the same memory operations, so that the
+  // compiler's wait counts are exact: ring m + 2's inputs are loaded two
+  // iterations ahead (the Sigma residual too, from the ghost ring's zero
+  // slots outside the float-pair band), and ring m - 2's outputs are
+  // stored (the iterations before the strip's first stored ring, and the
+  // residuals outside the band, write the never-read ghost-ring slots of
+  // the output arrays).  A sweep has one wave per column strip and fewer
+  // strips than SIMDs, so only its own prefetch hides memory latency; the
+  // three register sets P0..P2 take turns (ring m in one, ring m + 2
+  // loaded into the one ring m - 1 left) in a loop unrolled by three, so
+  // no register has to be renamed while its load is in flight (each set
+  // loads its own ring's vrad as v0, a cache hit: the previous ring loaded
+  // it as its v1).
+  const long S = (long)(nr + 1) * ns, ghost = (long)nr * ns;
+  auto lo_at = [&](int ring, long l) { return in_lob(ring, a.lob0, a.lob1) ? S + l : S + ghost + (l - (long)ring * ns); };
+  struct pset {
+    float s, v0, v1, vtp, vt, sl;
+  };
+  auto load = [&](pset &p, int mr) {
+    const int mp = min(mr, nr - 1);
+    const long l = (long)mp * ns;
+    p.s = a.fdens[l + j];
+    p.v0 = a.fvr[l + j];
+    p.v1 = a.fvr[l + ns + j];
+    p.vtp = a.fvt[l + jp];
+    p.vt = a.fvt[l + j];
+    p.sl = a.fdens[lo_at(mp, l + j)];
+  };
+  pset P0, P1, P2;
+  load(P0, m0);
+  load(P1, m0 + 1);
+  asm volatile("" ::"v"(P0.s), "v"(P0.v0), "v"(P0.v1), "v"(P0.vtp), "v"(P0.vt), "v"(P0.sl));
+  asm volatile("" ::"v"(P1.s), "v"(P1.v0), "v"(P1.v1), "v"(P1.vtp), "v"(P1.vt), "v"(P1.sl));
+  auto ring = [&](int m, pset &cur, pset &next2) {
     const int r = rt.at(t, m, nr);
     if (bg.r0 != rt.r0) bg.load(a.sbar, a.vref, rt.r0, nr);
-    const float c_s = n_s, c_v0 = n_v0, c_v1 = n_v1, c_vtp = n_vtp, c_vt = n_vt;
-    // the residual of ring m's Sigma: read only in LO iterations (rings
-    // outside LO iterations are outside the band)
-    const float c_sl = lo ? n_sl : 0.f;
-    {
-      const int mp = min(m + 1, nr - 1);
-      const long l = (long)mp * ns;
-      n_s = a.fdens[l + j];
-      n_v0 = c_v1;
-      n_v1 = a.fvr[l + ns + j];
-      n_vtp = a.fvt[l + jp];
-      n_vt = a.fvt[l + j];
-      n_sl = lo ? a.fdens[lo_at(mp, l + j)] : 0.f;
-    }
+    const float c_s = cur.s, c_v0 = cur.v0, c_v1 = cur.v1, c_vtp = cur.vtp, c_vt = cur.vt;
+    const float c_sl = cur.sl;  // 0 outside the band
+    load(next2, m + 2);
     for (int q = 0; q < NQ; q++) {
       Q[q][0] = Q[q][1];
       Q[q][1] = Q[q][2];
@@ -2212,8 +2212,9 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
       sg[2] = 0.0f;
       vr[2] = 0.0f;
     }
+    // (k = -1, the first step of the first strip: its slopes and fluxes are
+    // of zero quantities and feed no stored ring)
     const int k = m - 1;
-    if (!decltype(store)::value && k < 0) return;  // storing steps have k >= 1
     const bool flat = (k == 0 || k >= nr - 1);
     const float idm0 = flat ? 0.f : (float)lane_get(rt.idm, r - 1), idm1 = flat ? 0.f : (float)lane_get(rt.idm, r);
     for (int q = 0; q < NQ; q++) {
@@ -2238,10 +2239,12 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
       F[q][0] = F[q][1];
       F[q][1] = cf * QS[q];
     }
-    if (decltype(store)::value) {
-      // ring o = k - 1 is complete: stored as FP32 offsets from its backgrounds
+    {
+      // ring o = k - 1 is complete: stored as FP32 offsets from its
+      // backgrounds if the strip stores it (o in [b0, b1)), else into the
+      // ghost ring's slots, which nothing reads
       const int o = k - 1;
-      const long lo_ = (long)o * ns + j;
+      const long lo_ = (o >= b0 ? (long)o * ns : ghost) + j;
       const float isurf = (float)lane_get(rt.isurf, r - 2);
       const double sb = lane_get(bg.sb, r - 2);
       double val[NQ];
@@ -2249,29 +2252,27 @@ __device__ __forceinline__ void trad32_block(const trad_args &a, int blk) {
       // Sigma as the reader will reconstruct it, the angular momenta's background
       const double xs = val[4] - sb;
       const float fs = (float)xs, fsl = (float)(xs - (double)fs);
-      const float fsr = (lo && in_lob(o, a.lob0, a.lob1)) ? fsl : 0.f;
+      const bool pair = in_lob(o, a.lob0, a.lob1);
+      const float fsr = pair ? fsl : 0.f;
       const double lb = fp32_bg_angmom(rec32(sb, fs, fsr), lane_get(bg.vb, r - 2), lane_get(rt.Rmed, r - 2), OF);
       a.fout[0][lo_] = (float)val[0];
       a.fout[1][lo_] = (float)val[1];
       a.fout[2][lo_] = (float)(val[2] - lb);
       a.fout[3][lo_] = (float)(val[3] - lb);
       a.fout[4][lo_] = fs;
-      if (lo) a.fout[4][S + lo_] = fsl;  // read by k_taz only inside the band
+      // the residual of a float-pair ring (k_taz reads it inside the band only)
+      a.fout[4][S + (pair && o >= b0 ? lo_ : ghost + j)] = fsl;
     }
   };
-  // the LO iterations [L0, L1) of this strip (none when the band is empty)
-  const bool any_lo = a.lob0 < a.lob1;
-  const int L0 = any_lo ? a.lob0 - 2 : INT_MAX, L1 = any_lo ? a.lob1 + 2 : INT_MAX;
   int m = m0;
-  auto run = [&](int to, auto store) {
-    const int x0 = min(max(L0, m), to), x1 = min(max(L1, x0), to);
-    for (; m < x0; m++) ring(m, store, std::false_type());
-    for (; m < x1; m++) ring(m, store, std::true_type());
-    for (; m < to; m++) ring(m, store, std::false_type());
-  };
-  run(b0 + 2, std::false_type());
-  asm volatile("" ::"v"(n_s), "v"(n_v1), "v"(n_vtp), "v"(n_vt));
-  run(b1 + 2, std::true_type());
+  const int end = b1 + 2;  // steps m0 .. b1 + 1 store rings b0 .. b1 - 1
+  for (; m + 2 < end; m += 3) {
+    ring(m, P0, P2);
+    ring(m + 1, P1, P0);
+    ring(m + 2, P2, P1);
+  }
+  if (m < end) ring(m++, P0, P2);
+  if (m < end) ring(m++, P1, P0);
 }
 
 __global__ __launch_bounds__(TRAD_TPB) void k_trad32(trad_args a) { trad32_block(a, blockIdx.x); }
