@@ -240,12 +240,12 @@ def emulator_leg(n_eval, max_orbits):
     held-out draws (128 x 384 disks, the training seed + 1) evolved by the
     batched solver from both starts with the committed weights
     (cudaprotoplanetarydisks_amd/data/emulator_128x384.npz, trained on an
-    MI355X by the project's BF16-MFMA kernels on 512 batched runs of 300
+    MI355X by the project's BF16-MFMA kernels on 512 batched runs of 1000
     orbits).  Steady: the mass-weighted L1 change of the orbit-averaged ring
     profile per orbit below eps."""
     from cudaprotoplanetarydisks_amd.emulator import WEIGHTS, warm_start_leg
     t0 = time.perf_counter()
-    rep = warm_start_leg(WEIGHTS, n_eval, max_orbits, [2e-4, 1e-4, 5e-5])
+    rep = warm_start_leg(WEIGHTS, n_eval, max_orbits, [1e-4, 5e-5, 1e-5])
     rep["seconds"] = round(time.perf_counter() - t0, 1)
     return rep
 
@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--adsg-steps", type=int, default=50, help="substeps of the ADSG leg (0: skip)")
     ap.add_argument("--small-steps", type=int, default=1000, help="substeps of the 128x384 leg (0: skip)")
     ap.add_argument("--emulator-disks", type=int, default=32, help="held-out disks of the warm-start leg (0: skip)")
-    ap.add_argument("--emulator-orbits", type=int, default=300)
+    ap.add_argument("--emulator-orbits", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

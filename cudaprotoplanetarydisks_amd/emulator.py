@@ -98,25 +98,30 @@ def open_ensemble(params, directory, tiles=None):
     return Ensemble(decks, tiles=tiles)
 
 
-def generate(params, orbits, chunk=128, log=None):
+def generate(params, orbits, chunk=128, log=None, segment=100):
     """Run every disk `orbits` orbits from the plain initial state (batched,
-    `chunk` disks at a time); returns the output vectors, a row of NaN for
-    a disk whose run crashed (FG_E_CRASH: the batch stops that disk and
-    finishes the others)."""
+    `chunk` disks at a time, in runs of `segment` orbits with a progress
+    line after each: a disk stops at the first DT boundary at or after each
+    segment's end, 20 DT an orbit, so the segments change nothing); returns
+    the output vectors, a row of NaN for a disk whose run crashed
+    (FG_E_CRASH: the batch stops that disk and finishes the others)."""
     from .native import FG_E_CRASH, FargoError
     out = []
-    t_end = orbits * ORBIT
+    ends = [o for o in range(segment, int(math.ceil(orbits)), segment) if o < orbits] + [orbits]
     for c0 in range(0, len(params), chunk):
         ps = params[c0:c0 + chunk]
         with tempfile.TemporaryDirectory() as tmp:
             t0 = time.perf_counter()
             with open_ensemble(ps, tmp) as ens:
-                try:
-                    steps = ens.run(t_end)
-                except FargoError as e:
-                    if e.status != FG_E_CRASH:
-                        raise
-                    steps = None
+                steps = 0
+                for o_end in ends:
+                    try:
+                        steps += ens.run(o_end * ORBIT)
+                    except FargoError as e:
+                        if e.status != FG_E_CRASH:
+                            raise
+                    if log and o_end < orbits:
+                        log({"generating": c0 + len(ps), "orbit": o_end, "seconds": round(time.perf_counter() - t0, 1)})
                 for k, p in enumerate(ps):
                     if ens.get(k, "crashed"):
                         out.append(np.full(2 * NOUT, np.nan, np.float32))
@@ -381,15 +386,15 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--n-train", type=int, default=512)
     t.add_argument("--nrad", type=int, default=128)
-    t.add_argument("--orbits", type=float, default=300.0)
+    t.add_argument("--orbits", type=float, default=1000.0)
     t.add_argument("--steps", type=int, default=8000)
     t.add_argument("--seed", type=int, default=20261202)
     t.add_argument("--out", default=WEIGHTS)
     w = sub.add_parser("warmstart")
     w.add_argument("--weights", default=WEIGHTS)
-    w.add_argument("--n-eval", type=int, default=32)
-    w.add_argument("--max-orbits", type=int, default=300)
-    w.add_argument("--eps", type=float, nargs="+", default=[1e-3, 5e-4, 2e-4, 1e-4, 5e-5])
+    w.add_argument("--n-eval", type=int, default=128)
+    w.add_argument("--max-orbits", type=int, default=1200)
+    w.add_argument("--eps", type=float, nargs="+", default=[1e-4, 5e-5, 2e-5, 1e-5])
     s = sub.add_parser("study")
     s.add_argument("--n-train", type=int, default=512)
     s.add_argument("--n-eval", type=int, default=32)
