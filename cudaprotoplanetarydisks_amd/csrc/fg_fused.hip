@@ -46,6 +46,14 @@
 #define HALO_B 2  // rows below the tile ( .. i + TH + 1)
 #define HALO_C 4  // columns on each side
 #define HC (TW + 2 * HALO_C)
+// sector of a staged column j0 - HALO_C + c (j0 = the tile's first column):
+// at most one ring length outside [0, ns) once ns reaches the staged width,
+// so a compare and an add or subtract replace the two runtime modulos
+__device__ __forceinline__ int wrap_col(int x, int ns) {
+  if (x < 0) x += ns;
+  while (x >= ns) x -= ns;
+  return x;
+}
 #define SRC64_TPB 512  // k_src: two 80 KB workgroups per CU, four waves per SIMD
 
 fg_status fg_disk_force_async(fg_ctx *c, const double *dens, double x, double y, double rs, double *out_dev);
@@ -167,7 +175,7 @@ __global__ __launch_bounds__(KTPB) void k_src(src_args a) {
   const fg_dev_tables &t = a.t;
   const double dt = a.dt;
 #define GI(r) (i0 - HALO_R + (r))
-#define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
+#define GJ(c) wrap_col(j0 - HALO_C + (c), ns)
 
   // ---- S0: load, CorrectVtheta, damping #1, pressure, potential ----
   // all of a thread's loads are issued before any of them is used
@@ -1819,7 +1827,7 @@ __device__ __forceinline__ void src32_tile(const src32_args &a, int bx, int by, 
   const double dt = a.dt;
   const float dtf = (float)dt;
 #define GI(r) (i0 - HALO_R + (r))
-#define GJ(c) ((((j0 - HALO_C + (c)) % ns) + ns) % ns)
+#define GJ(c) wrap_col(j0 - HALO_C + (c), ns)
   // per-ring tables of the tile's rows as in k_src, plus the reference
   // vtheta (0 outside the disk), and the potential's cos/sin per column
   constexpr int RT_VREF = RT_N, RT_SBAR = RT_N + 1, MAXR = SRC32_MAX_TH + HALO_R + HALO_B;
