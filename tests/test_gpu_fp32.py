@@ -154,3 +154,15 @@ def test_fp32_pairs_rejects_bad_width(deck):
 def call_pairs(ctx, w):
     from cudaprotoplanetarydisks_amd.native import call
     call("fg_set_fp32_pairs", ctx, w)
+
+
+def test_fp32_two_planets_track_fp64(deck, tmp_path):
+    """Two planets (the float-pair band spans both planets' rings and the
+    rings between) against the FP64 run."""
+    from cudaprotoplanetarydisks_amd.workloads import write_planets
+    cfg = write_planets(tmp_path / "two.cfg", [("A", 1.0, 1e-3, 0.0, "YES", "NO"), ("B", 1.7, 5e-4, 0.0, "YES", "YES")])
+    d = deck(name="two.par", Nrad=64, Nsec=192, PlanetConfig=cfg)
+    for rep in compare(d, 8, every=4):
+        e = rep["errors"]
+        assert e["dens"]["linf"] <= DENS_LINF, rep
+        assert e["vtheta"]["linf"] <= VEL_LINF, rep
