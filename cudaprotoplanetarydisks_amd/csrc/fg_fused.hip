@@ -1547,82 +1547,6 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     }
     if (i == 0) return;
   } else {
-  // Rings of up to FIN_OV_K * 192 cells: lane 0 of wave 0 forms the ring
-  // mean while waves 1-3 load and evaluate the parts of each cell's time
-  // step that do not depend on it, the radial, sound-speed and shear terms
-  // (A = invdt1^2 + invdt2^2, B = invdt4^2 below), so the latency of the
-  // sequential sum overlaps the CFL pass's loads.  The sum of squares then
-  // completes as (A + invdt3^2) + B, the reference's left-to-right order.
-  if (FIN_B > 1 && ns <= 192 * FIN_OV_K) {
-    double A[FIN_OV_K], B[FIN_OV_K];
-    if (tid == 0) {
-      double m = seq_sum_lds(ring, ns);
-      m /= (double)ns;
-      a.vmoy[i] = m;
-      s_vmoy = m;
-    }
-    if (tid >= 64 && i > 0) {
-      const double invdt1r = cs / mindx;
-      const recip64 rdr(dxrad), rdt(dxtheta);
-#pragma unroll
-      for (int u0 = 0; u0 < FIN_OV_K; u0 += 4) {
-        double pvr[4], prp[4], prm[4], pd0[4], pd1[4], pcs[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int j = tid - 64 + (u0 + u) * 192;
-          pvr[u] = prp[u] = prm[u] = pd0[u] = pd1[u] = pcs[u] = 0.0;
-          if (j < ns) {
-            const long l = base + j;
-            pvr[u] = a.vr[l];
-            if (adi) pcs[u] = a.cs[l];
-            if (radial_n) {
-              prp[u] = a.xrmp[l];
-              prm[u] = a.xrmm[l + ns];
-              pd1[u] = a.xd[l + ns];
-              pd0[u] = a.xd[l];
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int j = tid - 64 + (u0 + u) * 192;
-          A[u0 + u] = B[u0 + u] = 0.0;
-          if (j >= ns) continue;
-          const double vr = pvr[u];
-          double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + pd0[u] + 1e-20) : 0.0;
-          if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
-          const double invdt1 = adi ? pcs[u] / mindx : invdt1r;
-          const double invdt2 = rdr.div(fabs(vr));
-          double dvr = vrn - vr;
-          double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
-          dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
-          dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
-          double x = rdr.div(dvr), y = rdt.div(dvt);
-          double invdt4 = (y > x) ? y : x;
-          invdt4 *= 4.0 * 1.41 * 1.41;
-          A[u0 + u] = invdt1 * invdt1 + invdt2 * invdt2;
-          B[u0 + u] = invdt4 * invdt4;
-        }
-      }
-    }
-    __syncthreads();
-    if (i == 0) return;
-    if (tid >= 64) {
-      const double vm = s_vmoy;
-      const recip64 rdt(dxtheta);
-#pragma unroll
-      for (int u = 0; u < FIN_OV_K; u++) {
-        const int j = tid - 64 + u * 192;
-        if (j >= ns) break;
-        const double vres = a.fast ? (ring[j] - vm) : ring[j];
-        const double invdt3 = rdt.div(fabs(vres));
-        const double dtc = 0.5 / sqrt((A[u] + invdt3 * invdt3) + B[u]);
-        if (dtc < best) best = dtc;
-      }
-    }
-    ring_min_store(best, red, a.newdt + i);
-    return;
-  }
   if (tid == 0) {
     double m = seq_sum_lds(ring, ns);
     m /= (double)ns;
@@ -1634,6 +1558,7 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
   vm = s_vmoy;
   }
   const recip64 rdr(dxrad), rdt(dxtheta);
+  double smax = -1.0;
   for (int j0 = tid; j0 < ns; j0 += FIN_B * 256) {
     double pvr[FIN_B], prp[FIN_B], prm[FIN_B], pd0[FIN_B], pd1[FIN_B], pcs[FIN_B];
 #pragma unroll
@@ -1682,9 +1607,186 @@ __device__ __forceinline__ void fin_ring(const fin_args &a, int i, double *ring)
     double x = rdr.div(dvr), y = rdt.div(dvt);
     double invdt4 = (y > x) ? y : x;
     invdt4 *= 4.0 * 1.41 * 1.41;
-    double dtc = 0.5 / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4);
+    const double sq = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4;
+    if (sq > smax) smax = sq;
+  }
+  }
+  // the smallest rounded 0.5 / sqrt(sq) is the one at the largest sq (both
+  // roundings are monotone; a NaN sq is skipped by both comparisons)
+  if (smax >= 0.0) {
+    const double dtc = 0.5 / sqrt(smax);
     if (dtc < best) best = dtc;
   }
+  ring_min_store(best, red, a.newdt + i);
+}
+
+// FP64 rings of up to FIN_OV_K * 192 cells.  The ring mean of vtheta is a
+// left-to-right sum in one lane (3072 dependent adds at 1024 x 3072), so it
+// starts at once: wave 0 loads the ring's transported vtheta, damps it as
+// the pass below does, stages it in LDS and lane 0 adds it up, while waves
+// 1-3 do everything else in one pass over their cells: ComputeVelocities
+// (Kernels.cu:1200-1205) with the energy, sound speed and temperature of an
+// adiabatic deck, and every term of the cell's CFL time step
+// (Kernels.cu:1284-1329) that does not need the mean.  Only the
+// vtheta-residual and shear terms wait for the workgroup barrier.
+// The ring minimum of dt = 0.5 / sqrt(S) over the cells is taken as one
+// quotient at the largest S: rounded sqrt and rounded division are both
+// monotone, so 0.5 / sqrt(max S) is the smallest rounded dt bit for bit
+// (a NaN S is skipped by both comparisons; the 1e30 start caps both).
+template <bool ADI>
+__device__ __forceinline__ void fin_ring_ov(const fin_args &a, int i, double *ring) {
+  constexpr int U = 2;  // cells per thread whose loads are issued together
+  __shared__ double red[4];
+  __shared__ double s_vmoy;
+  const int ns = a.ns, nr = a.nr, tid = threadIdx.x;
+  const fg_dev_tables &t = a.t;
+  const long base = (long)i * ns;
+  double lam_i = 0, vr0_i = 0, vt0_i = 0;
+  const bool damp_i = a.evanescent && t.ev_zone[i];
+  if (damp_i) ev_consts(t, a.nu, i, a.dt, a.omega_frame, lam_i, vr0_i, vt0_i, a.sgacc);
+  const bool cfl = i > 0;  // ring 0 has no CFL term (Kernels.cu:1286)
+  double A[FIN_OV_K], X[FIN_OV_K];
+  if (tid < 64) {
+    // the damped vtheta of every cell, which the pass below also stores
+    for (int j0 = 0; j0 < ns; j0 += 64 * 16) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int j = j0 + u * 64 + tid;
+        v[u] = j < ns ? a.xvt[base + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int j = j0 + u * 64 + tid;
+        if (j < ns) ring[j] = damp_i ? damp_toward(v[u], lam_i, vt0_i) : v[u];
+      }
+    }
+    // one wave: its LDS stores complete in order before lane 0's loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (tid == 0) {
+      double m = seq_sum_lds(ring, ns);
+      m /= (double)ns;
+      a.vmoy[i] = m;
+      s_vmoy = m;
+    }
+  } else {
+    double lam_n = 0, vr0_n = 0, vt0_n = 0;
+    const bool damp_n = a.evanescent && (i + 1 < nr) && t.ev_zone[i + 1];
+    if (damp_n) ev_consts(t, a.nu, i + 1, a.dt, a.omega_frame, lam_n, vr0_n, vt0_n, a.sgacc);
+    const bool radial = i > 0;          // vrad recovery reads rings i-1 and i
+    const bool radial_n = i + 1 < nr;  // vrad of ring i+1 (0 on the ghost ring)
+    const double dxrad = t.Rsup[i] - t.Rinf[i];
+    const double dxtheta = t.dxtheta_cfl[i];
+    const double mindx = (dxtheta < dxrad) ? dxtheta : dxrad;
+    const double invdt1r = t.cs_iso[i] / mindx;
+    const recip64 rdr(dxrad);
+    bool bad = false;
+    const int t0 = tid - 64;
+#pragma unroll
+    for (int u0 = 0; u0 < FIN_OV_K; u0 += U) {
+      double lrp[U], lrm[U], ld[U], ldm[U], lvt[U], lp[U], prp[U], prm[U], pd1[U], pcs[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = t0 + (u0 + u) * 192;
+        lrp[u] = lrm[u] = ld[u] = ldm[u] = lvt[u] = lp[u] = prp[u] = prm[u] = pd1[u] = pcs[u] = 0.0;
+        if (j < ns) {
+          const long l = base + j;
+          if (radial) {
+            lrp[u] = a.xrmp[l - ns];
+            lrm[u] = a.xrmm[l];
+            ldm[u] = a.xd[l - ns];
+          }
+          ld[u] = a.xd[l];
+          lvt[u] = a.xvt[l];
+          if (ADI)
+            lp[u] = a.xe[l];
+          else if (a.temp)
+            lp[u] = a.press[l];
+          if (cfl && radial_n) {
+            prp[u] = a.xrmp[l];
+            prm[u] = a.xrmm[l + ns];
+            pd1[u] = a.xd[l + ns];
+          }
+          if (ADI && cfl && !a.cs_refresh) pcs[u] = a.cs[l];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = t0 + (u0 + u) * 192;
+        if (j >= ns) break;
+        const long l = base + j;
+        // ComputeVelocities, Kernels.cu:1200-1205 (0 on ring 0)
+        double vr = radial ? (lrp[u] + lrm[u]) / (ld[u] + ldm[u] + 1e-20) : 0.0, vt = lvt[u], s = ld[u];
+        if (damp_i) {
+          vr = damp_toward(vr, lam_i, vr0_i);
+          vt = damp_toward(vt, lam_i, vt0_i);
+          s = damp_toward(s, lam_i, t.SigmaMed[i]);
+        }
+        a.dens[l] = s;
+        bad |= !(s >= 0.0 && s < HUGE_VAL);
+        a.vr[l] = vr;
+        a.vt[l] = vt;
+        double csv = pcs[u];
+        if (ADI) {
+          double e = lp[u];
+          if (damp_i) e = damp_toward(e, lam_i, t.EnergyMed[i]);
+          a.energy[l] = e;
+          bad |= !(e >= 0.0 && e < HUGE_VAL);
+          if (a.temp) a.temp[l] = (a.gamma - 1.0) * e / s;  // Kernels.cu:211
+          if (a.cs_refresh) {
+            csv = sqrt(a.gamma * (a.gamma - 1.0) * e / s);  // Kernels.cu:181
+            a.cs[l] = csv;
+          }
+        } else if (a.temp) {
+          a.temp[l] = lp[u] / s;
+        }
+        if (i == nr - 1) {  // ghost ring after transport: all zero
+          a.dens[l + ns] = 0.0;
+          a.vr[l + ns] = 0.0;
+          a.vt[l + ns] = 0.0;
+          if (ADI) a.energy[l + ns] = 0.0;
+        }
+        if (cfl) {
+          double vrn = radial_n ? (prp[u] + prm[u]) / (pd1[u] + ld[u] + 1e-20) : 0.0;
+          if (damp_n) vrn = damp_toward(vrn, lam_n, vr0_n);
+          const double invdt1 = ADI ? csv / mindx : invdt1r;
+          const double invdt2 = rdr.div(fabs(vr));
+          double dvr = vrn - vr;
+          dvr = (dvr >= 0.0) ? 1e-10 : -dvr;
+          A[u0 + u] = invdt1 * invdt1 + invdt2 * invdt2;
+          X[u0 + u] = rdr.div(dvr);
+        }
+      }
+    }
+    if (bad) *a.crash = 1;
+  }
+  __syncthreads();
+  if (i == 0) return;
+  double best = 1e30;
+  if (tid >= 64) {
+    const double vm = s_vmoy;
+    const recip64 rdt(t.dxtheta_cfl[i]);
+    double smax = -1.0;
+#pragma unroll
+    for (int u = 0; u < FIN_OV_K; u++) {
+      const int j = tid - 64 + u * 192;
+      if (j >= ns) break;
+      const double vres = a.fast ? (ring[j] - vm) : ring[j];
+      const double invdt3 = rdt.div(fabs(vres));
+      double dvt = ring[j + 1 == ns ? 0 : j + 1] - ring[j];
+      dvt = (dvt >= 0.0) ? 1e-10 : -dvt;
+      const double y = rdt.div(dvt);
+      double invdt4 = (y > X[u]) ? y : X[u];
+      invdt4 *= 4.0 * 1.41 * 1.41;
+      const double sq = (A[u] + invdt3 * invdt3) + invdt4 * invdt4;
+      if (sq > smax) smax = sq;
+    }
+    if (smax >= 0.0) {
+      const double dtc = 0.5 / sqrt(smax);
+      if (dtc < best) best = dtc;
+    }
   }
   ring_min_store(best, red, a.newdt + i);
 }
@@ -1705,6 +1807,8 @@ __global__ __launch_bounds__(256) void k_fin(fin_args a) {
       fin_ring<2, ADI, true, true>(a, i, lds_fin);
     else
       fin_ring<2, ADI, true, false>(a, i, lds_fin);
+  } else if (a.ns <= 192 * FIN_OV_K) {
+    fin_ring_ov<ADI>(a, i, lds_fin);
   } else {
     fin_ring<4, ADI, false>(a, i, lds_fin);
   }
