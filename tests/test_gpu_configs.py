@@ -100,6 +100,9 @@ def test_config2_fused_fp64_1024x3072_bitexact(tmp_path):
             sim.substep()
             orc.substep()
             assert int(sim.get("last_cfl")) == orc.iget("last_cfl"), k
+            # the disk's force on the star (DeviceReduce order) of this substep
+            for ax in ("IndirectTerm.x", "IndirectTerm.y"):
+                assert sim.get(ax) == orc.get(ax), (k, ax, sim.get(ax), orc.get(ax))
             assert np.array_equal(sim.ctx.nshift(), orc.ifield("Nshift")[:-1]), k
             d, vr, vt = sim.state()
             for g, name in ((d, "Dens"), (vr, "Vrad"), (vt, "Vtheta")):
