@@ -188,7 +188,7 @@ extern "C" fg_status fg_circumplanetary_mass(fg_ctx *c, double xpl, double ypl, 
 // ---- non-reflecting boundary (SideEuler.cu:45-100) -------------------
 // Ring gathers and their DeviceReduce means, then the two shifted-copy
 // kernels and the mean correction.
-__global__ void k_nr_copy(double *dens, double *energy, int i_angle, int i_angle2, int nr, int ns) {
+__global__ void k_nr_copy(double *dens, double *energy, int i_angle, int ns) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
   // ring 0 <- ring 1 shifted by i_angle (wrapped modulo NSEC)
@@ -259,7 +259,7 @@ fg_status fg_nonreflecting_internal(fg_ctx *c) {
   dangle2 *= (Rmed[i] - Rmed[i - 1]);
   int i_angle2 = (int)(dangle2 / 2.0 / PI * (double)ns + .5);
   dim3 g(fg_blocks(ns, 256)), b(256);
-  hipLaunchKernelGGL(k_nr_copy, g, b, 0, c->stream, dens, en, i_angle, i_angle2, nr, ns);
+  hipLaunchKernelGGL(k_nr_copy, g, b, 0, c->stream, dens, en, i_angle, ns);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nr_copy_outer, g, b, 0, c->stream, dens, en, i_angle2, nr, ns);
   FG_LAUNCH_CHECK();

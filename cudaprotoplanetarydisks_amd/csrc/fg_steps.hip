@@ -65,6 +65,7 @@ __global__ void k_temperature(double *T, const double *dens, const double *P, co
 // staged in LDS and summed sequentially by one lane (the reference's j
 // order) so the mean is bit-identical.
 __global__ void k_ring_mean(double *out, const double *field, int nr, int ns) {
+  (void)nr;  // one block per ring: the grid holds the ring count
   extern __shared__ double ring[];
   const int i = blockIdx.x;
   for (int j = threadIdx.x; j < ns; j += blockDim.x) ring[j] = field[(long)i * ns + j];
@@ -433,6 +434,7 @@ __global__ void k_compute_velocities(double *vr, double *vt, const double *dens,
 __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const double *vt, const double *cs,
                            const double *dens, const double *energy, int *crash, fg_dev_tables t, int nr, int ns,
                            int fast) {
+  (void)nr;  // one block per ring: the grid holds the ring count
   extern __shared__ double sh[];
   double *ring = sh;               // ns
   double *red = sh + ns;           // blockDim
@@ -484,7 +486,7 @@ __global__ void k_cfl_ring(double *vmoy, double *newdt, const double *vr, const 
   red[threadIdx.x] = best;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s && red[threadIdx.x + s] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+    if ((int)threadIdx.x < s && red[threadIdx.x + s] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) newdt[i] = red[0];
