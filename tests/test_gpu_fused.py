@@ -52,9 +52,11 @@ def test_fused_cases_bitexact(deck, case):
 
 
 @pytest.mark.parametrize("case", [
-    dict(Nrad=12, Nsec=3200),  # k_fin's ring pass without the overlapped CFL split (NSEC > 3072)
+    dict(Nrad=12, Nsec=3000),  # k_fin's single sweep with its last staging chunk and cells partial
+    dict(Nrad=10, Nsec=3072),  # the widest ring of k_fin's single sweep
+    dict(Nrad=12, Nsec=3200),  # k_fin's two-pass ring kernel (NSEC > 3072)
     dict(Nrad=70, Nsec=4096),  # the widest ring the fused path takes
-], ids=["nsec3200", "nsec4096"])
+], ids=["nsec3000", "nsec3072", "nsec3200", "nsec4096"])
 def test_fused_wide_rings_bitexact(deck, case):
     run_fused(deck(**case), 4)
 

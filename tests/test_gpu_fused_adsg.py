@@ -84,6 +84,8 @@ def run_fused_vs_oracle(path, nsub, tol, adiabatic):
 CASES = {
     "adiabatic": dict(Nrad=32, Nsec=96, DT=0.02, **ADI),
     "adiabatic_odd": dict(Nrad=37, Nsec=50, DT=0.02, **ADI),
+    # k_fin's single sweep with the energy, partial staging chunk and cells
+    "adiabatic_wide": dict(Nrad=12, Nsec=3000, DT=0.01, **ADI),
     "relax_cooling": dict(Nrad=32, Nsec=96, DT=0.02, Cooling="YES", CoolingTime0=62.8, **ADI),
     "beta_cooling": dict(Nrad=32, Nsec=96, DT=0.02, Cooling="YES", CoolingBeta=10.0, **ADI),
     "alpha_visc": dict(Nrad=32, Nsec=96, DT=0.02, Viscosity=None, AlphaViscosity=1e-3, **ADI),
