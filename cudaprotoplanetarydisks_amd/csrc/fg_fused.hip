@@ -876,6 +876,11 @@ struct taz_args {
 #define TAZW_HL 5
 #define TAZW_HR 4
 #define TAZW_P32 (64 * 4 - TAZW_HL - TAZW_HR)  // cells output per FP32 chunk (CL = 4)
+// FP64 chunks: three cells per lane (the five- and six-quantity sweeps fit
+// in 142 and 164 VGPRs, three waves per SIMD; four or five cells per lane
+// take 174-208 VGPRs and leave two)
+#define TAZW_CL64 3
+#define TAZW_P64 (64 * TAZW_CL64 - TAZW_HL - TAZW_HR)  // cells output per FP64 chunk
 
 // x of lane l-1 (lane 0 reads 0): DPP wave_shr:1 on each dword
 __device__ __forceinline__ double lane_prev(double x) {
@@ -1184,28 +1189,6 @@ __global__ __launch_bounds__(64) void k_tazw(taz_args a) {
     tazw_body<CL, F32, NQT, F32>(a, blockIdx.y, blockIdx.x, sh);
   else
     tazw_body<CL, F32, NQT, false>(a, blockIdx.y, blockIdx.x, sh);
-}
-
-// cells per lane of the wave kernel for a ring of ns cells: CL = 4 or 5
-// (two waves per SIMD without spills: 174 and 206 VGPRs; CL >= 6 drops to
-// one wave or spills), the fewer total cells (chunks x 64 x CL), then the
-// longer chunks.  At 1024 x 3072: CL = 5, 10 chunks, 4 % halo overhead;
-// k_taz 143 us as one 256-thread workgroup per ring, 84 us as wave chunks.
-static int tazw_cells(int ns, int *nchunks) {
-  int best = 4, bestn = 0;
-  long bestc = -1;
-  for (int cl = 4; cl <= 5; cl++) {
-    const int p = 64 * cl - TAZW_HL - TAZW_HR;
-    const int n = (ns + p - 1) / p;
-    const long cells = (long)n * 64 * cl;
-    if (bestc < 0 || cells <= bestc) {
-      bestc = cells;
-      best = cl;
-      bestn = n;
-    }
-  }
-  *nchunks = bestn;
-  return best;
 }
 
 // ------------------------------------------------------------ k_fin ----
@@ -2939,18 +2922,16 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
   za.lob0 = lob0;
   za.lob1 = lob1;
   prof_mark(c);
-  if (adi) {
-    // six quantities: CL = 4 keeps two waves per SIMD
-    const dim3 gw((ns + TAZW_P32 - 1) / TAZW_P32, nr);
-    hipLaunchKernelGGL((k_tazw<4, false, 6>), gw, dim3(64), 0, c->stream, za);
-  } else if (!fp32) {
-    int nch = 0;
-    const int cl = tazw_cells(ns, &nch);
-    const dim3 gw(nch, nr);
-    if (cl == 4)
-      hipLaunchKernelGGL((k_tazw<4, false>), gw, dim3(64), 0, c->stream, za);
+  if (!fp32) {
+    // FP64, five quantities or six with the energy: at 1024 x 3072, 17
+    // chunks a ring (6 % halo cells) at three waves per SIMD run 65 us
+    // against 72 us for 10 chunks of five cells per lane at two waves (the
+    // ring as one 256-thread workgroup took 143 us)
+    const dim3 gw((ns + TAZW_P64 - 1) / TAZW_P64, nr);
+    if (adi)
+      hipLaunchKernelGGL((k_tazw<TAZW_CL64, false, 6>), gw, dim3(64), 0, c->stream, za);
     else
-      hipLaunchKernelGGL((k_tazw<5, false>), gw, dim3(64), 0, c->stream, za);
+      hipLaunchKernelGGL((k_tazw<TAZW_CL64, false>), gw, dim3(64), 0, c->stream, za);
   } else {
     // FP32: CL = 4 (112 VGPRs, four waves per SIMD); 113 -> 56 us at
     // 1024 x 3072 against one 256-thread workgroup per ring
