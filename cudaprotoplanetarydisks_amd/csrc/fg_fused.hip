@@ -875,7 +875,11 @@ struct taz_args {
 // overlap by those halos; each cell is output by exactly one chunk.
 #define TAZW_HL 5
 #define TAZW_HR 4
-#define TAZW_P32 (64 * 4 - TAZW_HL - TAZW_HR)  // cells output per FP32 chunk (CL = 4)
+#define TAZW_P32 (64 * 4 - TAZW_HL - TAZW_HR)  // cells output per batched FP32 chunk (CL = 4)
+// single-disk FP32 chunks: three cells per lane in their own kernel held to
+// 96 VGPRs, five waves per SIMD (seven dwords go to scratch)
+#define TAZW_CL32 3
+#define TAZW_P32S (64 * TAZW_CL32 - TAZW_HL - TAZW_HR)
 // FP64 chunks: three cells per lane (the five- and six-quantity sweeps fit
 // in 142 and 164 VGPRs, three waves per SIMD; four or five cells per lane
 // take 174-208 VGPRs and leave two)
@@ -1189,6 +1193,15 @@ __global__ __launch_bounds__(64) void k_tazw(taz_args a) {
     tazw_body<CL, F32, NQT, F32>(a, blockIdx.y, blockIdx.x, sh);
   else
     tazw_body<CL, F32, NQT, false>(a, blockIdx.y, blockIdx.x, sh);
+}
+
+// the FP32 bulk mode's single-disk sweep (TAZW_CL32 cells per lane)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_tazw32(taz_args a) {
+  __shared__ tazw_lds<TAZW_CL32, 5> sh;
+  if (in_lob(blockIdx.y, a.lob0, a.lob1))
+    tazw_body<TAZW_CL32, true, 5, true>(a, blockIdx.y, blockIdx.x, sh);
+  else
+    tazw_body<TAZW_CL32, true, 5, false>(a, blockIdx.y, blockIdx.x, sh);
 }
 
 // ------------------------------------------------------------ k_fin ----
@@ -2933,10 +2946,11 @@ extern "C" fg_status fg_step_fused(fg_ctx *c, double dt, double omega_frame, dou
     else
       hipLaunchKernelGGL((k_tazw<TAZW_CL64, false>), gw, dim3(64), 0, c->stream, za);
   } else {
-    // FP32: CL = 4 (112 VGPRs, four waves per SIMD); 113 -> 56 us at
-    // 1024 x 3072 against one 256-thread workgroup per ring
-    const dim3 gw((ns + TAZW_P32 - 1) / TAZW_P32, nr);
-    hipLaunchKernelGGL((k_tazw<4, true>), gw, dim3(64), 0, c->stream, za);
+    // FP32: three cells per lane at five waves per SIMD, 40.7 -> 38.4 us at
+    // 1024 x 3072 against four cells per lane at four (113 us as one
+    // 256-thread workgroup per ring)
+    const dim3 gw((ns + TAZW_P32S - 1) / TAZW_P32S, nr);
+    hipLaunchKernelGGL(k_tazw32, gw, dim3(64), 0, c->stream, za);
   }
   FG_LAUNCH_CHECK();
   // k_fin
